@@ -1,0 +1,150 @@
+/*
+ * wind_amd C-ABI: batched stiff-ODE integration (semi-implicit Euler and
+ * RK2 step doubling with per-system timestep control) on AMD Instinct
+ * MI355X (gfx950).
+ *
+ * Two layers:
+ *
+ * 1. Drop-in entry points of agurvich/WIND, exported with C++ linkage so the
+ *    mangled names match what WIND's Python frontend looks up through ctypes
+ *    (python/wind_harness.py:18-54).  They live in per-network shim libraries
+ *    wind_amd/lib/<network>/{sie,rk2,sie_host}.so, mirroring WIND's
+ *    cuda/lib/{sie,rk2,sie_host}.so rebuilt per network:
+ *
+ *      int cudaIntegrateSystem(float tnow, float tend, int n_integration_steps,
+ *                              float *constants, float *equations, int Nsystems,
+ *                              int Nequations_per_system, float ABSOLUTE,
+ *                              float RELATIVE);
+ *          symbol _Z19cudaIntegrateSystemffiPfS_iiff in sie.so and rk2.so.
+ *          Replaces cuda/common/harness.cu:5-106 (one launch of the fused
+ *          kernel cuda/common/kernel.cu:140-310).  Results follow the CPU
+ *          c_baseline (cuda/c_baseline/common/common_gold.c:8-153), bit for
+ *          bit.  Returns the total sub-step count (3 per attempt, summed over
+ *          systems), truncated to int like the reference's int counter.
+ *
+ *      int cudaIntegrateSIE(<same 9 arguments>);
+ *          symbol _Z16cudaIntegrateSIEffiPfS_iiff in sie_host.so.  Replaces
+ *          cuda/SIE_host/harness_kernel.cu:427-532 (lockstep SIE: one
+ *          timestep for all systems).  Returns nsteps * Nsystems.
+ *
+ *      void initializeCublasExternally(void);
+ *          symbol _Z26initializeCublasExternallyv in sie_host.so.  Replaces
+ *          the symbol the reference's harness calls before cudaIntegrateSIE
+ *          (python/wind_harness.py:27,139); here it initialises the device.
+ *
+ * 2. The extended C API below (extern "C"): explicit networks, device-resident
+ *    state, per-system step counts and status, 64-bit counters, error text.
+ */
+#ifndef WIND_AMD_WIND_H
+#define WIND_AMD_WIND_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes returned by the extended API. */
+enum wind_error {
+    WIND_OK = 0,
+    WIND_ERR_ARGUMENT = -1,   /* invalid argument; see wind_last_error() */
+    WIND_ERR_DEVICE = -2,     /* HIP runtime error */
+    WIND_ERR_UNSUPPORTED = -3 /* network / N / kernel combination not built */
+};
+
+enum wind_method { WIND_SIE = 0, WIND_RK2 = 1 };
+
+/* Arithmetic mode.  EXACT reproduces the reference c_baseline's float
+ * arithmetic operation for operation (explicit Gauss-Jordan inverse, no
+ * fused multiply-add, double-precision relative error test), so results
+ * are bit-identical to it.  FAST solves (I - hJ) x = f by LU elimination
+ * with fused multiply-adds and hardware exp/log; results differ from the
+ * reference by rounding (and, through step acceptance, by trajectory). */
+enum wind_mode { WIND_MODE_EXACT = 0, WIND_MODE_FAST = 1 };
+
+/* Kernel family.  AUTO picks by network and N. */
+enum wind_kernel {
+    WIND_KERNEL_AUTO = 0,
+    WIND_KERNEL_BLOCK = 1,  /* one workgroup per system, equation per thread
+                               (the reference's launch shape) */
+    WIND_KERNEL_LANE = 2    /* one system per lane, persistent work queue */
+};
+
+/* Per-system termination status (extension; the reference loops forever
+ * where STALLED or EXHAUSTED is reported). */
+enum wind_status {
+    WIND_STATUS_OK = 0,
+    WIND_STATUS_STALLED = 1,   /* an accepted step did not advance tnow and
+                                  the next trial would repeat it exactly */
+    WIND_STATUS_EXHAUSTED = 2  /* max_attempts reached */
+};
+
+/* Built-in networks. */
+enum wind_builtin {
+    WIND_NET_KATZ96 = 0,      /* N = 5, 10 constants per system */
+    WIND_NET_NR_TEST = 1,     /* N = 2, 4 constants per system */
+    WIND_NET_HHE6 = 2,        /* N = 6, 1 constant (n_H); 24 network params */
+    WIND_NET_MASS_ACTION = 3  /* N species, R reactions a+b->c+d; 1 constant */
+};
+
+typedef struct wind_network wind_network;
+
+/* Creates a handle for a compiled-in network.  `params` is required for
+ * WIND_NET_HHE6 (24 floats: lnA[7], b[7], C[7], Gamma[3]) and ignored
+ * otherwise. */
+int wind_network_create_builtin(int builtin, const float *params, int nparams,
+                                wind_network **out);
+
+/* Creates a mass-action network: reaction r is rx[4r] + rx[4r+1] ->
+ * rx[4r+2] + rx[4r+3] with rate k[r] * c[0] * y_a * y_b.  The library copies
+ * the table and builds its device-side gather lists. */
+int wind_network_create_mass_action(int N, int R, const int32_t *rx, const float *k,
+                                    wind_network **out);
+
+void wind_network_destroy(wind_network *net);
+int wind_network_size(const wind_network *net); /* N */
+
+typedef struct wind_options {
+    int method;            /* wind_method */
+    int mode;              /* wind_mode */
+    int kernel;            /* wind_kernel */
+    int const_stride;      /* floats per system in `constants`; 0 = network default */
+    int device_pointers;   /* 1: constants, equations, steps_out, status_out are
+                              device pointers (state stays resident) */
+    int64_t max_attempts;  /* per-system attempt cap; 0 = default (2^24) */
+    int32_t *steps_out;    /* optional [Nsystems]: 3 * attempts per system */
+    int32_t *status_out;   /* optional [Nsystems]: wind_status */
+    int64_t *total_steps;  /* optional: total sub-steps, 64-bit */
+    void *stream;          /* hipStream_t; NULL = the library's stream */
+    float *kernel_ms;      /* optional: device time of the integration kernel */
+} wind_options;
+
+void wind_options_init(wind_options *opts);
+
+/* Integrates every system from tnow to tend (in place in `equations`),
+ * with the initial trial step (tend - tnow) / n_integration_steps.
+ * Synchronous: returns after results are in `equations`. */
+int wind_integrate(const wind_network *net, float tnow, float tend, int n_integration_steps,
+                   const float *constants, float *equations, int Nsystems, int Neqn,
+                   float ABSOLUTE, float RELATIVE, const wind_options *opts);
+
+/* Lockstep SIE with the semantics of the reference's cudaIntegrateSIE
+ * (cuda/SIE_host/harness_kernel.cu:248-425): one shared timestep, any failing
+ * system halves it for all, doubling after acceptance, return after 9
+ * consecutive failures.  Host pointers.  *result = 3 * attempts * Nsystems.
+ * Built for the compiled networks (not mass action). */
+int wind_integrate_lockstep(const wind_network *net, float tnow, float tend,
+                            int n_integration_steps, const float *constants,
+                            float *equations, int Nsystems, int Neqn, float ABSOLUTE,
+                            float RELATIVE, int const_stride, int64_t *result);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *wind_last_error(void);
+
+/* Library version string. */
+const char *wind_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

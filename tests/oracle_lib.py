@@ -1,0 +1,180 @@
+"""ctypes access to the oracle (oracle/_build/libwind_oracle.so) and, where it
+was built, the reference's own c_baseline (oracle/_ref/{sie,rk2}_gold.so).
+Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libwind_oracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION = 0, 1, 2, 3
+SIE, RK2 = 0, 1
+
+
+class OracleNet(C.Structure):
+    _fields_ = [("id", C.c_int), ("N", C.c_int), ("params", C.c_void_p), ("R", C.c_int),
+                ("rx", C.c_void_p), ("k", C.c_void_p)]
+
+
+class OracleTrace(C.Structure):
+    _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("tnow", C.c_void_p), ("h", C.c_void_p),
+                ("flag", C.c_void_p), ("y1", C.c_void_p), ("y2", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        L = C.CDLL(ORACLE_SO)
+        vp, i, f, d = C.c_void_p, C.c_int, C.c_float, C.c_double
+        L.oracle_gold_integrate.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp]
+        L.oracle_gold_integrate_ex.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, C.c_longlong, vp]
+        L.oracle_gold_integrate_mt.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, i]
+        L.oracle_gold_integrate_f64.argtypes = [i, vp, d, d, i, vp, i, vp, i, i, d, d, vp]
+        L.oracle_integrate_system_traced.argtypes = [i, vp, f, f, f, vp, vp, i, f, f, vp, C.c_longlong, vp]
+        L.oracle_gje_invert.argtypes = [vp, vp, i]
+        L.oracle_gje_invert.restype = None
+        L.oracle_matvec.argtypes = [vp, vp, i]
+        L.oracle_matvec.restype = None
+        L.oracle_check_error.argtypes = [i, vp, vp, i, f, f]
+        L.oracle_take_step.argtypes = [i, vp, f, f, i, vp, vp, i]
+        for n in ("oracle_rhs", "oracle_jac"):
+            getattr(L, n).restype = None
+        L.oracle_rhs.argtypes = [vp, f, vp, vp, vp]
+        L.oracle_jac.argtypes = [vp, vp, vp, vp]
+        L.oracle_rhs_f64.argtypes = [vp, d, vp, vp, vp]
+        L.oracle_rhs_f64.restype = None
+        L.oracle_jac_f64.argtypes = [vp, vp, vp, vp]
+        L.oracle_jac_f64.restype = None
+        L.oracle_wexpf.argtypes = [f]
+        L.oracle_wexpf.restype = f
+        L.oracle_wlogf.argtypes = [f]
+        L.oracle_wlogf.restype = f
+        _lib = L
+    return _lib
+
+
+class Net:
+    """Oracle network descriptor; keeps its arrays alive."""
+
+    def __init__(self, ident, N, params=None, rx=None, k=None):
+        self.params = None if params is None else np.ascontiguousarray(params, np.float32)
+        self.rx = None if rx is None else np.ascontiguousarray(rx, np.int32).reshape(-1)
+        self.k = None if k is None else np.ascontiguousarray(k, np.float32)
+        self.N = N
+        self.s = OracleNet(ident, N, None if self.params is None else self.params.ctypes.data,
+                           0 if self.k is None else len(self.k),
+                           None if self.rx is None else self.rx.ctypes.data,
+                           None if self.k is None else self.k.ctypes.data)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def katz96():
+    return Net(NET_KATZ96, 5)
+
+
+def nr_test():
+    return Net(NET_NR_TEST, 2)
+
+
+def hhe6(params):
+    return Net(NET_HHE6, 6, params=params)
+
+
+def mass_action(N, rx, k):
+    return Net(NET_MASS_ACTION, N, rx=rx, k=k)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def gold_integrate(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL,
+                   const_stride=None, max_attempts=0, threads=1):
+    """Returns (total, steps[nsys], status[nsys]); `equations` updated in place."""
+    L = lib()
+    stride = len(constants) // max(nsys, 1) if const_stride is None else const_stride
+    steps = np.zeros(nsys, np.int32)
+    status = np.zeros(nsys, np.int32)
+    if threads > 1:
+        tot = L.oracle_gold_integrate_mt(method, net.ref, tnow, tend, nfix, constants.ctypes.data, stride,
+                                         equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data,
+                                         threads)
+    else:
+        tot = L.oracle_gold_integrate_ex(method, net.ref, tnow, tend, nfix, constants.ctypes.data, stride,
+                                         equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data,
+                                         max_attempts, status.ctypes.data)
+    return tot, steps, status
+
+
+def gold_integrate_f64(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL, const_stride=None):
+    L = lib()
+    stride = len(constants) // max(nsys, 1) if const_stride is None else const_stride
+    steps = np.zeros(nsys, np.int32)
+    tot = L.oracle_gold_integrate_f64(method, net.ref, tnow, tend, nfix, constants.ctypes.data, stride,
+                                      equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data)
+    return tot, steps
+
+
+def traced(method, net, tnow, tend, h, y, c, ABS, REL, capacity=100000, max_attempts=1 << 22):
+    """One system with its attempt trace; returns (substeps, y_final, trace, status)."""
+    L = lib()
+    N = net.N
+    tr_t = np.zeros(capacity, np.float32)
+    tr_h = np.zeros(capacity, np.float32)
+    tr_f = np.zeros(capacity, np.int32)
+    tr_y1 = np.zeros(capacity * N, np.float32)
+    tr_y2 = np.zeros(capacity * N, np.float32)
+    tr = OracleTrace(capacity, 0, tr_t.ctypes.data, tr_h.ctypes.data, tr_f.ctypes.data,
+                     tr_y1.ctypes.data, tr_y2.ctypes.data)
+    y = _f32(y).copy()
+    c = _f32(c)
+    st = C.c_int32(0)
+    n = L.oracle_integrate_system_traced(method, net.ref, tnow, tend, h, y.ctypes.data, c.ctypes.data, N,
+                                         ABS, REL, C.byref(tr), max_attempts, C.byref(st))
+    k = min(tr.count, capacity)
+    return n, y, dict(tnow=tr_t[:k], h=tr_h[:k], flag=tr_f[:k], y1=tr_y1[:k * N].reshape(k, N),
+                      y2=tr_y2[:k * N].reshape(k, N)), st.value
+
+
+# ----------------------------------------------------------------- reference
+def ref_available():
+    return all(os.path.exists(os.path.join(REF_DIR, n)) for n in ("sie_gold.so", "rk2_gold.so"))
+
+
+_ref = {}
+
+
+def ref(method):
+    """The reference c_baseline built from source by oracle/ref.mk (SIE or RK2 library)."""
+    if method not in _ref:
+        L = C.CDLL(os.path.join(REF_DIR, "sie_gold.so" if method == SIE else "rk2_gold.so"))
+        vp, i, f = C.c_void_p, C.c_int, C.c_float
+        L.goldIntegrateSystem.argtypes = [f, f, i, vp, vp, i, i, f, f]
+        L.integrateSystem.argtypes = [f, f, f, vp, vp, vp, vp, i, f, f]
+        L.checkError.argtypes = [vp, vp, i, f, f]
+        L.take_step.argtypes = [f, f, i, vp, vp, vp, vp, vp, i]
+        L.calculate_dydt.argtypes = [f, vp, vp, vp, i]
+        L.calculate_dydt.restype = None
+        L.calculate_jacobian.argtypes = [vp, vp, vp]
+        L.calculate_jacobian.restype = None
+        if method == SIE:
+            L.gjeInvertMatrix.argtypes = [vp, vp, i]
+            L.gjeInvertMatrix.restype = None
+            L.matrixVectorMult.argtypes = [vp, vp, i]
+            L.matrixVectorMult.restype = None
+        _ref[method] = L
+    return _ref[method]

@@ -1,0 +1,142 @@
+"""GPU parity: the HIP integrators (through the C-ABI) against the oracle.
+
+Exact mode must reproduce the oracle bit for bit (final states, per-system
+step counts, status), NaN positions included.  Sizes are chosen so the
+oracle finishes in seconds.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from wind_amd import api
+from wind_amd import networks as W
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def same(a, b):
+    """Bitwise equality of float32 arrays, treating every NaN as equal to every NaN."""
+    a = np.asarray(a, np.float32).reshape(-1)
+    b = np.asarray(b, np.float32).reshape(-1)
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a[~na].view(np.uint32), b[~nb].view(np.uint32)) or np.array_equal(a[~na], b[~nb])
+
+
+def first_diff(a, b):
+    a = np.asarray(a, np.float32).reshape(-1)
+    b = np.asarray(b, np.float32).reshape(-1)
+    bad = np.where(~((a == b) | (np.isnan(a) & np.isnan(b))))[0]
+    return None if len(bad) == 0 else (int(bad[0]), float(a[bad[0]]), float(b[bad[0]]), len(bad))
+
+
+def run_gpu(net, tnow, tend, nfix, c, y, nsys, tol, method, kernel, mode=api.MODE_EXACT, max_attempts=0):
+    y = y.copy()
+    steps = np.zeros(nsys, np.int32)
+    status = np.zeros(nsys, np.int32)
+    total, _ = api.integrate(net, tnow, tend, nfix, c, y, nsys, tol, tol, method=method, mode=mode,
+                             kernel=kernel, steps_out=steps, status_out=status, max_attempts=max_attempts)
+    return y, steps, status, total
+
+
+def run_oracle(onet, tnow, tend, nfix, c, y, nsys, tol, method, stride, max_attempts=0):
+    y = y.copy()
+    total, steps, status = O.gold_integrate(method, onet, tnow, tend, nfix, c, y, nsys, tol, tol,
+                                            const_stride=stride, max_attempts=max_attempts)
+    return y, steps, status, total
+
+
+def check(gpu, ref, label):
+    y, s, st, tot = gpu
+    yr, sr, str_, totr = ref
+    print(label, "substeps gpu", tot, "oracle", totr, flush=True)
+    assert np.array_equal(st, str_), label
+    assert np.array_equal(s, sr), (label, np.where(s != sr)[0][:5])
+    assert same(y, yr), (label, first_diff(y, yr))
+    assert tot == int(np.sum(sr.astype(np.int64)))
+
+
+KERNELS = [api.KERNEL_LANE, api.KERNEL_BLOCK]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_nr_test_exact(kernel, method):
+    """Config 1 workload shape: NR_test, t in [0, 1], dt0 = 1e-3, tol 1e-3."""
+    y0, c, _ = W.nr_test_systems(1024, seed=3)
+    net = api.Network.builtin(api.NET_NR_TEST)
+    g = run_gpu(net, 0.0, 1.0, 1000, c, y0, 1024, 1e-3, method, kernel)
+    r = run_oracle(O.nr_test(), 0.0, 1.0, 1000, c, y0, 1024, 1e-3, method, 4)
+    check(g, r, f"nr_test m{method} k{kernel}")
+
+
+def katz_inputs(n, seed):
+    from golden.make_golden import katz_random_state
+
+    ys, cs = katz_random_state(np.random.default_rng(seed), n)
+    return ys.reshape(-1).copy(), cs.reshape(-1).copy()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+@pytest.mark.parametrize("tol", [5e-3, 1e-5])
+def test_katz96_exact(kernel, method, tol):
+    """Katz96 with 10 constants per system; attempt cap 3000 on both sides so
+    systems on which explicit RK2 never settles end EXHAUSTED identically."""
+    n = 300
+    y0, c = katz_inputs(n, seed=11)
+    net = api.Network.builtin(api.NET_KATZ96)
+    g = run_gpu(net, 0.0, 50.0, 1, c, y0, n, tol, method, kernel, max_attempts=3000)
+    r = run_oracle(O.katz96(), 0.0, 50.0, 1, c, y0, n, tol, method, 10, max_attempts=3000)
+    check(g, r, f"katz96 m{method} k{kernel} tol{tol}")
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_hhe6_exact(kernel, method):
+    """hhe6 (config 2 network) over 1 kyr; includes systems whose first
+    1-kyr attempt overflows to NaN, which both sides must reproduce."""
+    n = 256
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(n, seed=5)
+    net = api.Network.builtin(api.NET_HHE6, p)
+    g = run_gpu(net, 0.0, 1.0e3, 1, nH, y0, n, 1e-3, method, kernel, max_attempts=4000)
+    r = run_oracle(O.hhe6(p), 0.0, 1.0e3, 1, nH, y0, n, 1e-3, method, 1, max_attempts=4000)
+    check(g, r, f"hhe6 m{method} k{kernel}")
+
+
+@pytest.mark.parametrize("N", [5, 10, 32])
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_mass_action_exact(N, method):
+    n = 64 if N < 32 else 24
+    rx, k = W.mass_action_network(N, seed=100 + N)
+    y0, sc = W.mass_action_systems(n, N, seed=200 + N, scale_range=(0.3, 3.0))
+    net = api.Network.mass_action(N, rx, k)
+    tend = W.MASS_ACTION_TEND if method == api.SIE else 1.0e5
+    g = run_gpu(net, 0.0, tend, 1, sc, y0, n, 1e-3, method, api.KERNEL_AUTO, max_attempts=3000)
+    r = run_oracle(O.mass_action(N, rx, k), 0.0, tend, 1, sc, y0, n, 1e-3, method, 1, max_attempts=3000)
+    check(g, r, f"mass_action N{N} m{method}")
+
+
+def test_legacy_symbol_reference_fixture():
+    """The drop-in symbol, loaded as WIND's harness loads it, on the reference's
+    own smoke-test fixture (cuda/debug/inputKatz96_1_1_1.h, NUM_CONST = 2),
+    against the reference c_baseline's recorded output."""
+    from golden.make_golden import KATZ_FIXTURE_C, KATZ_FIXTURE_EQ
+
+    G = np.load(os.path.join(ROOT, "tests", "golden", "c_baseline_kats.npz"))
+    for lib, name in (("sie.so", "sie"), ("rk2.so", "rk2")):
+        so = C.CDLL(os.path.join(ROOT, "wind_amd", "lib", "katz96_numconst2", lib))
+        fn = getattr(so, "_Z19cudaIntegrateSystemffiPfS_iiff")
+        fn.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                       C.c_float]
+        for tend in (10, 200):
+            eq = KATZ_FIXTURE_EQ.copy()
+            n = fn(0.0, float(tend), 1, KATZ_FIXTURE_C.ctypes.data, eq.ctypes.data, 4, 5, 5e-3, 5e-3)
+            assert n == int(G[f"fixture_t{tend}_{name}_n"]), (name, tend, n)
+            assert same(eq, G[f"fixture_t{tend}_{name}_eq"]), (name, tend)

@@ -1,0 +1,246 @@
+// Integrator kernels for table-driven mass-action networks (any N up to
+// kMassActionMaxN): one workgroup per system, state and network values in
+// LDS, the matrix in LDS when it fits and otherwise in a per-workgroup slice
+// of a device scratch buffer.
+//
+// Network evaluation is a gather: each reaction's rate w_r = ((k_r s) y_a) y_b
+// and partials da_r = (k_r s) y_b, db_r = (k_r s) y_a are computed once (one
+// thread per reaction), then each species' f_i and each Jacobian non-zero is
+// summed from an ordered contribution list built on the host.  The order of
+// every list is the order in which a serial loop over reactions (ascending r,
+// updating a, b, c, d; columns a then b) touches that element, so each sum
+// rounds exactly like the serial loop.
+//
+// Exact mode runs the reference's Gauss-Jordan inverse
+// (cuda/c_baseline/linear_algebra/gje_gold.c:56-144) with the updates of one
+// pivot spread over (row, column) pairs - every element still sees the same
+// sequence of operations - followed by the transposed mat-vec
+// (matrix_ops.c:6-27).  Fast mode factors (I - hJ) by unpivoted LU with
+// fused multiply-adds and substitutes.
+#pragma once
+#include "wind_device.h"
+#include "wind_internal.h"
+
+namespace wind {
+
+struct MALayout {
+    float *y, *yw, *y1, *f, *g, *w, *da, *db;
+    float *M, *Inv;
+};
+
+__device__ inline void ma_eval(const MassActionDevice &net, const float *yv, float scale, MALayout &L,
+                               float nh, bool want_jac) {
+    const int N = net.N, R = net.R, T = blockDim.x, t = threadIdx.x;
+    for (int r = t; r < R; r += T) {
+        const int a = net.rx[4 * r], b = net.rx[4 * r + 1];
+        const float ks = net.k[r] * scale;
+        const float ya = yv[a], yb = yv[b];
+        L.w[r] = (ks * ya) * yb;
+        L.da[r] = ks * yb;
+        L.db[r] = ks * ya;
+    }
+    __syncthreads();
+    for (int i = t; i < N; i += T) {
+        float acc = 0.0f;
+        for (int e = net.f_ptr[i]; e < net.f_ptr[i + 1]; e++) {
+            const int ent = net.f_ent[e];
+            const float v = L.w[ent >> 1];
+            acc = (ent & 1) ? acc - v : acc + v;
+        }
+        L.f[i] = acc;
+    }
+    if (!want_jac) return;
+    // M = (-h) * J with J zero outside its non-zeros (sie_gold.c:17-36),
+    // then the diagonal + 1 in a second pass.
+    const float zs = nh * 0.0f;
+    for (int e = t; e < N * N; e += T) L.M[e] = zs;
+    __syncthreads();
+    for (int z = t; z < net.nnz; z += T) {
+        float acc = 0.0f;
+        for (int e = net.j_ptr[z]; e < net.j_ptr[z + 1]; e++) {
+            const int ent = net.j_ent[e];
+            const int r = ent >> 2;
+            const float v = (ent & 2) ? L.db[r] : L.da[r];
+            acc = (ent & 1) ? acc - v : acc + v;
+        }
+        L.M[net.j_pos[z]] = nh * acc;
+    }
+    __syncthreads();
+    for (int i = t; i < N; i += T) L.M[i * (N + 1)] = L.M[i * (N + 1)] + 1.0f;
+    __syncthreads();
+}
+
+// Exact Gauss-Jordan inverse and mat-vec; on return g = inverse * f.
+__device__ inline void block_gje_exact(int N, MALayout &L) {
+    const int T = blockDim.x, t = threadIdx.x;
+    float *M = L.M, *Inv = L.Inv;
+    for (int e = t; e < N * N; e += T) Inv[e] = (e % (N + 1) == 0) ? 1.0f : 0.0f;
+    __syncthreads();
+    for (int r = 0; r < N; r++) {
+        const float s = M[r * N + r];
+        for (int c = t; c < N; c += T) {
+            if (c > r)
+                M[r * N + c] = M[r * N + c] / s;
+            else
+                Inv[r * N + c] = Inv[r * N + c] / s;
+        }
+        __syncthreads();
+        const int rows = N - 1 - r;
+        for (int idx = t; idx < rows * N; idx += T) {
+            const int q = r + 1 + idx / N, c = idx % N;
+            const float sq = M[q * N + r];
+            if (c > r)
+                M[q * N + c] = M[q * N + c] - sq * M[r * N + c];
+            else
+                Inv[q * N + c] = Inv[q * N + c] - sq * Inv[r * N + c];
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < N; r++) {
+        const int b = N - 1 - r;
+        // rows bq < b, processed in the reference's order bq = b-1, b-2, ...;
+        // the rows are independent, so any assignment to threads is exact.
+        for (int idx = t; idx < b * N; idx += T) {
+            const int bq = idx / N, c = idx % N;
+            Inv[bq * N + c] = Inv[bq * N + c] - M[bq * N + b] * Inv[b * N + c];
+        }
+        __syncthreads();
+    }
+    for (int r = t; r < N; r += T) {
+        float acc = 0.0f;
+        for (int c = 0; c < N; c++) acc = acc + Inv[c * N + r] * L.f[c];
+        L.g[r] = acc;
+    }
+    __syncthreads();
+}
+
+// Fast LU solve of A x = f, A[i][j] = M[j*N + i]; x in L.g.
+__device__ inline void block_lu_fast(int N, MALayout &L) {
+    const int T = blockDim.x, t = threadIdx.x;
+    float *M = L.M;
+    float *b = L.g;
+    for (int i = t; i < N; i += T) b[i] = L.f[i];
+    __syncthreads();
+    for (int k = 0; k < N; k++) {
+        const float ip = __builtin_amdgcn_rcpf(M[k * N + k]);
+        for (int i = k + 1 + t; i < N; i += T) M[k * N + i] = M[k * N + i] * ip;  // l_i
+        __syncthreads();
+        const int rows = N - 1 - k;
+        // trailing update A(i,j) -= l_i A(k,j) for i, j > k; b_i -= l_i b_k
+        for (int idx = t; idx < rows * (rows + 1); idx += T) {
+            const int jj = idx / rows, i = k + 1 + idx % rows;
+            const float l = M[k * N + i];
+            if (jj < rows) {
+                const int j = k + 1 + jj;
+                M[j * N + i] = fmaf(-l, M[j * N + k], M[j * N + i]);
+            } else {
+                b[i] = fmaf(-l, b[k], b[i]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = N - 1; j >= 0; j--) {
+        if (t == 0) b[j] = b[j] * __builtin_amdgcn_rcpf(M[j * N + j]);
+        __syncthreads();
+        const float xj = b[j];
+        for (int i = t; i < j; i += T) b[i] = fmaf(-M[j * N + i], xj, b[i]);
+        __syncthreads();
+    }
+}
+
+template <int METHOD, bool FAST>
+__global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassActionDevice net,
+                                                          float *gscratch, int mat_in_lds) {
+    extern __shared__ float lds[];
+    const int N = net.N, R = net.R, t = threadIdx.x, T = blockDim.x;
+    MALayout L;
+    L.y = lds;
+    L.yw = L.y + N;
+    L.y1 = L.yw + N;
+    L.f = L.y1 + N;
+    L.g = L.f + N;
+    L.w = L.g + N;
+    L.da = L.w + R;
+    L.db = L.da + R;
+    float *mats = mat_in_lds ? (L.db + R) : gscratch + (size_t)blockIdx.x * 2 * N * N;
+    L.M = mats;
+    L.Inv = mats + N * N;
+    __shared__ int s_flag;
+    unsigned long long steps = 0;
+    for (int sid = blockIdx.x; sid < a.nsys; sid += gridDim.x) {
+        for (int i = t; i < N; i += T) L.y[i] = a.equations[(size_t)sid * N + i];
+        const float scale = a.constants[(size_t)sid * a.const_stride];
+        __syncthreads();
+        float tnow = a.tnow, h = a.h0;
+        int unsolved = 0, attempts = 0, status = WIND_STATUS_OK;
+        while (tnow < a.tend) {
+            const float hh = fminf(a.tend - tnow, h);
+            const float tb = tnow + hh;
+            const float hs_full = tb - tnow, hs_half = (tb - tnow) / 2.0f;
+            for (int pass = 0; pass < 3; pass++) {
+                const float hs = pass == 0 ? hs_full : hs_half;
+                if (pass != 2) {
+                    for (int i = t; i < N; i += T) L.yw[i] = L.y[i];
+                    __syncthreads();
+                }
+                ma_eval(net, L.yw, scale, L, -hs, METHOD == WIND_SIE);
+                if (METHOD == WIND_SIE) {
+                    if (FAST)
+                        block_lu_fast(N, L);
+                    else
+                        block_gje_exact(N, L);
+                    for (int i = t; i < N; i += T)
+                        L.yw[i] = FAST ? fmaf(hs, L.g[i], L.yw[i]) : L.yw[i] + hs * L.g[i];
+                } else {
+                    __syncthreads();
+                    for (int i = t; i < N; i += T)
+                        L.yw[i] = FAST ? fmaf(hs, L.f[i], L.yw[i]) : L.yw[i] + hs * L.f[i];
+                }
+                __syncthreads();
+                if (pass == 0)
+                    for (int i = t; i < N; i += T) L.y1[i] = L.yw[i];
+            }
+            if (t == 0) s_flag = 0;
+            __syncthreads();
+            bool fail = false;
+            for (int i = t; i < N; i += T)
+                fail = fail || species_fails<METHOD>(L.y1[i], L.yw[i], a.ABS, a.REL);
+            if (fail) s_flag = 1;
+            __syncthreads();
+            const bool flag = s_flag != 0;
+            attempts++;
+            if (flag && unsolved < 10) {
+                h = hh * 0.5f;
+                unsolved++;
+            } else {
+                for (int i = t; i < N; i += T)
+                    L.y[i] = (METHOD == WIND_RK2) ? 2.0f * L.yw[i] - L.y1[i] : L.yw[i];
+                const float tprev = tnow;
+                tnow = tnow + hh;
+                h = (METHOD == WIND_RK2) ? hh * 2.0f : a.tend - tnow;
+                unsolved = 0;
+                if (tnow == tprev && (METHOD == WIND_SIE || h == 0.0f)) {
+                    status = WIND_STATUS_STALLED;
+                    __syncthreads();
+                    break;
+                }
+            }
+            __syncthreads();
+            if (attempts >= a.max_attempts && tnow < a.tend) {
+                status = WIND_STATUS_EXHAUSTED;
+                break;
+            }
+        }
+        __syncthreads();
+        for (int i = t; i < N; i += T) a.equations[(size_t)sid * N + i] = L.y[i];
+        if (t == 0) {
+            if (a.steps_out) a.steps_out[sid] = 3 * attempts;
+            if (a.status_out) a.status_out[sid] = status;
+            steps += 3ull * (unsigned long long)attempts;
+        }
+        __syncthreads();
+    }
+    if (t == 0 && steps) atomicAdd(a.counters + 1, steps);
+}
+
+}  // namespace wind

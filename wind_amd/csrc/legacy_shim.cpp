@@ -1,0 +1,100 @@
+// Drop-in entry points of agurvich/WIND's GPU libraries, compiled once per
+// network and integrator like the reference's per-network rebuild
+// (python/shell_scripts/precompile_system.sh:7-26, cuda/Makefile:47-60).
+// C++ linkage on purpose: WIND's Python frontend looks the functions up by
+// their mangled names (python/wind_harness.py:27-40).
+//
+// Build-time definitions:
+//   WIND_SHIM_KIND       1 = sie.so, 2 = rk2.so, 3 = sie_host.so
+//   WIND_SHIM_NETWORK    a wind_builtin value
+//   WIND_SHIM_NUM_CONST  constant stride per system (the reference's NUM_CONST)
+//   WIND_SHIM_PARAMS     header defining wind_shim_params[] (hhe6) or the
+//                        reaction table wind_shim_rx[] / wind_shim_k[] and
+//                        WIND_SHIM_N / WIND_SHIM_R (mass action)
+#include <cstdio>
+#include <mutex>
+
+#include "../../include/wind.h"
+
+#ifdef WIND_SHIM_PARAMS
+#include WIND_SHIM_PARAMS
+#endif
+
+namespace {
+
+wind_network *shim_network() {
+    static std::once_flag once;
+    static wind_network *net = nullptr;
+    std::call_once(once, [] {
+        int rc;
+#if WIND_SHIM_NETWORK == 3
+        rc = wind_network_create_mass_action(WIND_SHIM_N, WIND_SHIM_R, wind_shim_rx, wind_shim_k, &net);
+#elif WIND_SHIM_NETWORK == 2
+        rc = wind_network_create_builtin(WIND_SHIM_NETWORK, wind_shim_params, 24, &net);
+#else
+        rc = wind_network_create_builtin(WIND_SHIM_NETWORK, nullptr, 0, &net);
+#endif
+        if (rc) {
+            std::fprintf(stderr, "wind_amd: network setup failed: %s\n", wind_last_error());
+            net = nullptr;
+        }
+    });
+    return net;
+}
+
+int shim_integrate(int method, float tnow, float tend, int n_integration_steps, float *constants,
+                   float *equations, int Nsystems, int Nequations_per_system, float ABSOLUTE,
+                   float RELATIVE) {
+    wind_network *net = shim_network();
+    if (!net) return -1;
+    wind_options o;
+    wind_options_init(&o);
+    o.method = method;
+    o.mode = WIND_MODE_EXACT;
+    o.const_stride = WIND_SHIM_NUM_CONST;
+    int64_t total = 0;
+    o.total_steps = &total;
+    int rc = wind_integrate(net, tnow, tend, n_integration_steps, constants, equations, Nsystems,
+                            Nequations_per_system, ABSOLUTE, RELATIVE, &o);
+    if (rc) {
+        std::fprintf(stderr, "wind_amd: %s\n", wind_last_error());
+        return -1;
+    }
+    // The reference accumulates into an int (cuda/common/kernel.cu:307-309).
+    return (int)(unsigned int)(uint64_t)total;
+}
+
+}  // namespace
+
+#if WIND_SHIM_KIND == 1 || WIND_SHIM_KIND == 2
+// cuda/common/harness.cu:5-106
+int cudaIntegrateSystem(float tnow, float tend, int n_integration_steps, float *constants,
+                        float *equations, int Nsystems, int Nequations_per_system, float ABSOLUTE,
+                        float RELATIVE) {
+    return shim_integrate(WIND_SHIM_KIND == 1 ? WIND_SIE : WIND_RK2, tnow, tend, n_integration_steps,
+                          constants, equations, Nsystems, Nequations_per_system, ABSOLUTE, RELATIVE);
+}
+#endif
+
+#if WIND_SHIM_KIND == 3
+// cuda/SIE_host/harness_kernel.cu:427-532 (lockstep SIE)
+int cudaIntegrateSIE(float tnow, float tend, int n_integration_steps, float *constants,
+                     float *equations, int Nsystems, int Nequations_per_system, float ABSOLUTE,
+                     float RELATIVE) {
+    wind_network *net = shim_network();
+    if (!net) return -1;
+    int64_t result = 0;
+    int rc = wind_integrate_lockstep(net, tnow, tend, n_integration_steps, constants, equations,
+                                     Nsystems, Nequations_per_system, ABSOLUTE, RELATIVE,
+                                     WIND_SHIM_NUM_CONST, &result);
+    if (rc) {
+        std::fprintf(stderr, "wind_amd: %s\n", wind_last_error());
+        return -1;
+    }
+    return (int)(unsigned int)(uint64_t)result;
+}
+
+// The reference initialises cuBLAS here (python/wind_harness.py:27,139);
+// this library needs no BLAS handle, so it only sets up the network.
+void initializeCublasExternally() { (void)shim_network(); }
+#endif

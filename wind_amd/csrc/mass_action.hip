@@ -1,0 +1,135 @@
+// Host side of the mass-action networks: gather-list construction, device
+// upload, and kernel launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels_mass_action.h"
+#include "wind_internal.h"
+
+using namespace wind;
+
+namespace {
+
+int upload(void **dst, const void *src, size_t bytes) {
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    if (src) {
+        e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    }
+    return WIND_OK;
+}
+
+struct Scratch {
+    float *ptr = nullptr;
+    size_t bytes = 0;
+};
+Scratch g_scratch;  // per-process matrix scratch; callers hold the context lock
+
+}  // namespace
+
+int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const float *k) {
+    n->rx.assign(rx, rx + 4 * (size_t)R);
+    n->k.assign(k, k + R);
+    // Contribution lists in the order of a serial sweep over reactions.
+    std::vector<std::vector<int32_t>> flist(N);
+    std::map<int, std::vector<int32_t>> jlist;
+    for (int r = 0; r < R; r++) {
+        const int32_t *x = rx + 4 * r;
+        for (int slot = 0; slot < 4; slot++) {
+            const int neg = slot < 2 ? 1 : 0;
+            flist[x[slot]].push_back(r * 2 + neg);
+            jlist[x[0] * N + x[slot]].push_back(((r * 2 + 0) * 2) + neg);
+            jlist[x[1] * N + x[slot]].push_back(((r * 2 + 1) * 2) + neg);
+        }
+    }
+    std::vector<int32_t> f_ptr(N + 1, 0), f_ent;
+    for (int i = 0; i < N; i++) {
+        f_ent.insert(f_ent.end(), flist[i].begin(), flist[i].end());
+        f_ptr[i + 1] = (int32_t)f_ent.size();
+    }
+    std::vector<int32_t> j_pos, j_ptr(1, 0), j_ent;
+    for (auto &kv : jlist) {
+        j_pos.push_back(kv.first);
+        j_ent.insert(j_ent.end(), kv.second.begin(), kv.second.end());
+        j_ptr.push_back((int32_t)j_ent.size());
+    }
+    MassActionDevice &d = n->ma;
+    d.N = N;
+    d.R = R;
+    d.nnz = (int)j_pos.size();
+    int rc = upload((void **)&d.rx, rx, sizeof(int32_t) * 4 * R);
+    if (!rc) rc = upload((void **)&d.k, k, sizeof(float) * R);
+    if (!rc) rc = upload((void **)&d.f_ptr, f_ptr.data(), sizeof(int32_t) * f_ptr.size());
+    if (!rc) rc = upload((void **)&d.f_ent, f_ent.data(), sizeof(int32_t) * f_ent.size());
+    if (!rc) rc = upload((void **)&d.j_pos, j_pos.data(), sizeof(int32_t) * j_pos.size());
+    if (!rc) rc = upload((void **)&d.j_ptr, j_ptr.data(), sizeof(int32_t) * j_ptr.size());
+    if (!rc) rc = upload((void **)&d.j_ent, j_ent.data(), sizeof(int32_t) * j_ent.size());
+    return rc;
+}
+
+void mass_action_release(wind_network *n) {
+    MassActionDevice &d = n->ma;
+    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    d = MassActionDevice();
+}
+
+template <int METHOD>
+int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
+                       int kernel, hipStream_t st) {
+    if (kernel == WIND_KERNEL_LANE)
+        return wind_fail(WIND_ERR_UNSUPPORTED, "mass-action networks run on the block kernel");
+    const MassActionDevice &d = net->ma;
+    const int N = d.N;
+    const bool fast = mode == WIND_MODE_FAST;
+    const int T = N <= 64 ? 64 : 256;
+    int dev = 0;
+    hipGetDevice(&dev);
+    int max_lds = 0;
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const size_t base = sizeof(float) * (5 * (size_t)N + 3 * (size_t)d.R);
+    const size_t mats = sizeof(float) * (fast ? 1 : 2) * (size_t)N * N;
+    const bool in_lds = base + mats + 64 <= (size_t)max_lds;
+    const size_t lds = base + (in_lds ? mats : 0);
+    if (lds + 64 > (size_t)max_lds) return wind_fail(WIND_ERR_UNSUPPORTED, "network too large for LDS");
+    auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, T, lds);
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    per_cu = std::max(per_cu, 1);
+    const int grid = (int)std::max(1LL, std::min((long long)num_cu * per_cu, (long long)a.nsys));
+    float *scratch = nullptr;
+    if (!in_lds) {
+        const size_t need = sizeof(float) * 2 * (size_t)N * N * grid;
+        if (need > g_scratch.bytes) {
+            if (g_scratch.ptr) (void)hipFree(g_scratch.ptr);
+            g_scratch.ptr = nullptr;
+            g_scratch.bytes = 0;
+            e = hipMalloc((void **)&g_scratch.ptr, need);
+            if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+            g_scratch.bytes = need;
+        }
+        scratch = g_scratch.ptr;
+    }
+    hipLaunchKernelGGL(kfn, dim3(grid), dim3(T), lds, st, a, d, scratch, in_lds ? 1 : 0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    return WIND_OK;
+}
+
+template int launch_mass_action<WIND_SIE>(int, const wind_network *, const IntegrateArgs &, int, int,
+                                          hipStream_t);
+template int launch_mass_action<WIND_RK2>(int, const wind_network *, const IntegrateArgs &, int, int,
+                                          hipStream_t);

@@ -1,0 +1,110 @@
+"""Seeded synthetic networks and initial conditions for the benchmark workloads.
+
+Networks:
+  nr_test      NR eq. 16.6.1 stiff linear pair (python/ode_systems/NR_test.py:34-38),
+               per-system constants a * [998, 1998, -999, -1999].
+  katz96       the reference's shipped H/He network (cuda/ode_system/ode_gold.c:3-93).
+  hhe6         H/He ionisation + internal energy, rates A T^b exp(-C/T) (see
+               wind_amd/csrc/wind_device.h, NetHHe6, for the equations).
+  mass_action  N species, R = 4N two-body reactions a + b -> c + d.
+
+All generators take an explicit seed (numpy PCG64) and return float32 arrays
+laid out as the reference ABI expects: equations[s*N + i], constants[s*C + k].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+HHE6_DEFAULT_SEED = 20190101
+MASS_ACTION_K_RANGE = (1e-18, 1e-6)
+MASS_ACTION_Y0_RANGE = (1e-12, 1.0)
+# Integration interval of the mass-action workloads, in the network's time unit.
+MASS_ACTION_TEND = 1.0e7
+HHE6_TEND_YR = 1.0e6
+HE_OVER_H = 0.08
+
+
+def _logu(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
+
+
+# --------------------------------------------------------------------- NR_test
+def nr_test_systems(nsys: int, seed: int = 0):
+    """Config 1: y0 = (1, 0), constants a*[998, 1998, -999, -1999], a ~ U[0.9, 1.1]."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.9, 1.1, nsys).astype(np.float32)
+    base = np.array([998.0, 1998.0, -999.0, -1999.0], dtype=np.float32)
+    constants = (a[:, None] * base[None, :]).astype(np.float32)
+    y0 = np.tile(np.array([1.0, 0.0], dtype=np.float32), (nsys, 1))
+    return y0.reshape(-1).copy(), constants.reshape(-1).copy(), a
+
+
+def nr_test_exact(a, t):
+    """Closed form (NR_test.py:72-77 with the per-system scale a)."""
+    a = np.asarray(a, dtype=np.float64)
+    y1 = 2.0 * np.exp(-a * t) - np.exp(-1000.0 * a * t)
+    y2 = -np.exp(-a * t) + np.exp(-1000.0 * a * t)
+    return np.stack([y1, y2], axis=-1)
+
+
+# --------------------------------------------------------------------- hhe6
+def hhe6_params(seed: int = HHE6_DEFAULT_SEED) -> np.ndarray:
+    """Network parameters [lnA[7], b[7], C[7], Gamma[3]] (config 2)."""
+    rng = np.random.default_rng(seed)
+    A = _logu(rng, 1e-13, 1e-9, 7)
+    b = rng.uniform(-0.8, 0.5, 7)
+    C = rng.uniform(0.0, 6.0e5, 7)
+    G = _logu(rng, 1e-16, 1e-12, 3)
+    return np.concatenate([np.log(A), b, C, G]).astype(np.float32)
+
+
+def hhe6_systems(nsys: int, seed: int = 1):
+    """Per-system initial state and constants (config 2).
+
+    T ~ logU[1e2, 1e8] K, n_H ~ logU[1e-4, 1e4] cm^-3, He/H = 0.08.  The
+    hydrogen ionised fraction is U[0, 1]; helium is split over He0/He+/He++
+    by a flat Dirichlet draw.  e = 1.5 mu T / 1e4 with mu the particles per H.
+    """
+    rng = np.random.default_rng(seed)
+    T = _logu(rng, 1e2, 1e8, nsys)
+    nH = _logu(rng, 1e-4, 1e4, nsys)
+    fH = rng.uniform(0.0, 1.0, nsys)
+    he = rng.dirichlet([1.0, 1.0, 1.0], nsys) * HE_OVER_H
+    y = np.zeros((nsys, 6), dtype=np.float64)
+    y[:, 0] = 1.0 - fH
+    y[:, 1] = fH
+    y[:, 2:5] = he
+    xe = y[:, 1] + y[:, 3] + 2.0 * y[:, 4]
+    mu = y[:, 0:5].sum(axis=1) + xe
+    y[:, 5] = 1.5 * mu * T / 1.0e4
+    return y.astype(np.float32).reshape(-1).copy(), nH.astype(np.float32).copy()
+
+
+# --------------------------------------------------------------- mass action
+def mass_action_network(N: int, seed: int, R: int | None = None):
+    """Random two-body network: R = 4N reactions, species uniform, k ~ logU[1e-18, 1e-6]."""
+    rng = np.random.default_rng(seed)
+    R = 4 * N if R is None else R
+    rx = rng.integers(0, N, size=(R, 4)).astype(np.int32)
+    k = _logu(rng, *MASS_ACTION_K_RANGE, R).astype(np.float32)
+    return rx.reshape(-1).copy(), k
+
+
+def mass_action_systems(nsys: int, N: int, seed: int, scale_range=(1.0, 1.0)):
+    """y0 ~ logU[1e-12, 1] per species; constants[s] = rate scale ~ logU(scale_range)."""
+    rng = np.random.default_rng(seed)
+    y0 = _logu(rng, *MASS_ACTION_Y0_RANGE, (nsys, N)).astype(np.float32)
+    lo, hi = scale_range
+    scale = (_logu(rng, lo, hi, nsys) if hi > lo else np.full(nsys, lo)).astype(np.float32)
+    return y0.reshape(-1).copy(), scale
+
+
+# ------------------------------------------------------------------- mixed N
+MIXED_SIZES = (5, 10, 32, 64, 157)
+MIXED_WEIGHTS = (0.40, 0.25, 0.15, 0.12, 0.08)
+
+
+def mixed_population(nsys: int, seed: int):
+    """Config 4: N drawn from MIXED_SIZES with MIXED_WEIGHTS; returns the N of each system."""
+    rng = np.random.default_rng(seed)
+    return rng.choice(np.array(MIXED_SIZES), size=nsys, p=np.array(MIXED_WEIGHTS))
