@@ -62,6 +62,17 @@ enum wind_method { WIND_SIE = 0, WIND_RK2 = 1 };
  * reference by rounding (and, through step acceptance, by trajectory). */
 enum wind_mode { WIND_MODE_EXACT = 0, WIND_MODE_FAST = 1 };
 
+/* Timestep controller.  REFERENCE is c_baseline's (common_gold.c:116-148):
+ * halve on failure (at most 10 times in a row, then accept anyway); after
+ * acceptance SIE retries the whole remaining interval and RK2 doubles the
+ * step; SIE keeps the two-half-step result, RK2 the extrapolation 2 y2 - y1.
+ * ADAPTIVE keeps the same step-doubling pair but controls the step from the
+ * error norm err = max_i |y2_i - y1_i| / (ABSOLUTE + RELATIVE |y2_i|)
+ * (evaluated in double): accept iff err <= 1 (a non-finite norm fails),
+ * keep the extrapolation 2 y2 - y1, and scale the step by
+ * clamp(0.9 / sqrt(err), 0.2, 4). */
+enum wind_controller { WIND_CTRL_REFERENCE = 0, WIND_CTRL_ADAPTIVE = 1 };
+
 /* Kernel family.  AUTO picks by network and N. */
 enum wind_kernel {
     WIND_KERNEL_AUTO = 0,
@@ -75,7 +86,8 @@ enum wind_kernel {
 enum wind_status {
     WIND_STATUS_OK = 0,
     WIND_STATUS_STALLED = 1,   /* an accepted step did not advance tnow and
-                                  the next trial would repeat it exactly */
+                                  the next trial would repeat it exactly,
+                                  or the step underflowed to zero */
     WIND_STATUS_EXHAUSTED = 2  /* max_attempts reached */
 };
 
@@ -117,6 +129,7 @@ typedef struct wind_options {
     int64_t *total_steps;  /* optional: total sub-steps, 64-bit */
     void *stream;          /* hipStream_t; NULL = the library's stream */
     float *kernel_ms;      /* optional: device time of the integration kernel */
+    int controller;        /* wind_controller */
 } wind_options;
 
 void wind_options_init(wind_options *opts);

@@ -139,7 +139,9 @@ void nr_test_jac(const R *, const R *c, R *J) {
 //   3 H+ +e->H0, 4 He+ +e->He0, 5 He+ +e->He0 (dielectronic), 6 He++ +e->He+
 // Photoionisation Gamma_0..2 of H0, He0, He+.
 // Energy: photoheating minus collisional-ionisation, recombination and
-// free-free cooling.  T = 1e4 e / (1.5 mu), floored at 10 K.
+// free-free cooling.  T_raw = 1e4 e / (1.5 mu); rates use T = max(T_raw, 10 K)
+// and the cooling is scaled by phi = min(1, T_raw / 10 K), so it vanishes as
+// e -> 0.
 namespace hhe6 {
 constexpr float YR = 3.15576e7f;
 constexpr float TUNIT = 1.0e4f;
@@ -152,8 +154,8 @@ constexpr float RECC = 0.75f;                            // recombination coolin
 
 template <class R>
 struct HHe6State {
-    R xe, mu, T, lnT, invT, nHyr, sqT;
-    R Rj[7], gj[7], G[3], dT[6];
+    R xe, mu, T, lnT, invT, nHyr, sqT, phi;
+    R Rj[7], gj[7], G[3], dT[6], dphi[6];
     bool clamped;
 };
 
@@ -163,9 +165,10 @@ void hhe6_state(const R *y, const R *c, const float *p, HHe6State<R> &s) {
     s.xe = (y[1] + y[3]) + (y[4] + y[4]);
     s.mu = (((y[0] + y[1]) + (y[2] + y[3])) + y[4]) + s.xe;
     R dTde = (R)TUNIT / ((R)1.5f * s.mu);
-    R T = y[5] * dTde;
-    s.clamped = !(T > (R)TFLOOR);
-    if (s.clamped) T = (R)TFLOOR;
+    R Traw = y[5] * dTde;
+    s.clamped = !(Traw > (R)TFLOOR);
+    R T = s.clamped ? (R)TFLOOR : Traw;
+    s.phi = s.clamped ? Traw * (R)0.1f : (R)1.0f;
     s.T = T;
     s.lnT = Math<R>::lg(T);
     s.invT = (R)1.0f / T;
@@ -177,13 +180,15 @@ void hhe6_state(const R *y, const R *c, const float *p, HHe6State<R> &s) {
     }
     for (int j = 0; j < 3; j++) s.G[j] = (R)p[21 + j] * (R)YR;
     s.sqT = Math<R>::ex((R)0.5f * s.lnT);
-    if (s.clamped) {
-        for (int k = 0; k < 6; k++) s.dT[k] = (R)0.0f;
-    } else {
-        R Tmu = T / s.mu;
-        const R dmu[5] = {1, 2, 1, 2, 3};
-        for (int k = 0; k < 5; k++) s.dT[k] = -(Tmu * dmu[k]);
-        s.dT[5] = dTde;
+    // derivatives of T_raw; the rates see them above the floor, phi below it
+    R Tmu = Traw / s.mu;
+    const R dmu[5] = {1, 2, 1, 2, 3};
+    R dTraw[6];
+    for (int k = 0; k < 5; k++) dTraw[k] = -(Tmu * dmu[k]);
+    dTraw[5] = dTde;
+    for (int k = 0; k < 6; k++) {
+        s.dT[k] = s.clamped ? (R)0.0f : dTraw[k];
+        s.dphi[k] = s.clamped ? dTraw[k] * (R)0.1f : (R)0.0f;
     }
 }
 
@@ -212,7 +217,7 @@ void hhe6_rhs(const R *y, const R *c, const float *p, R *f) {
     R crec = ((w3 + w4) + w5) * ((R)RECC * T4);
     R ion = (y[1] + y[3]) + (R)4.0f * y[4];
     R cff = (((R)LFF * s.nHyr) * s.sqT) * (xe * ion);
-    f[5] = heat - ((cci + crec) + cff);
+    f[5] = heat - ((cci + crec) + cff) * s.phi;
 }
 
 // Partial derivatives of w = (R xe + G) y_s with respect to y_k.
@@ -258,6 +263,9 @@ void hhe6_jac(const R *y, const R *c, const float *p, R *J) {
     R LNs = LN * s.sqT;
     R xion = xe * ion;
     R dsqc = ((R)0.5f * s.sqT) * s.invT;
+    R cci = ((((s.Rj[0] * xe) * y[0]) * (R)CHI0 + ((s.Rj[1] * xe) * y[2]) * (R)CHI1) +
+             ((s.Rj[2] * xe) * y[3]) * (R)CHI2);
+    R cool = (cci + W * recT) + LNs * xion;
     const R dxe[6] = {0, 1, 0, 1, 2, 0};
     const R dion[6] = {0, 1, 0, 1, 4, 0};
     for (int k = 0; k < 6; k++) {
@@ -274,7 +282,7 @@ void hhe6_jac(const R *y, const R *c, const float *p, R *J) {
         R dcci = (dc0[k] * (R)CHI0 + dc1[k] * (R)CHI1) + dc2[k] * (R)CHI2;
         R dcrec = ((dw3[k] + dw4[k]) + dw5[k]) * recT + W * ((R)RECC * (s.dT[k] * (R)1.0e-4f));
         R dcff = (LN * (dsqc * s.dT[k])) * xion + LNs * (dxe[k] * ion + xe * dion[k]);
-        col[5] = heat - ((dcci + dcrec) + dcff);
+        col[5] = heat - (((dcci + dcrec) + dcff) * s.phi + cool * s.dphi[k]);
     }
 }
 
@@ -469,10 +477,15 @@ void record(oracle_trace *tr, int n, R tnow, R h, int flag, const R *y1, const R
 // cases): an accepted step that does not advance tnow stops the system with
 // ORACLE_STATUS_STALLED when the next trial would repeat it exactly (SIE, or
 // RK2 with h == 0); max_attempts > 0 stops it with ORACLE_STATUS_EXHAUSTED.
+//
+// Controller ORACLE_CTRL_ADAPTIVE (an extension, not in the reference) keeps
+// the step-doubling pair and replaces the timestep logic: err = max_i
+// |y2 - y1| / (ABS + REL |y2|) in double, accept iff err <= 1 (NaN fails),
+// accepted value 2 y2 - y1, next step h * clamp(0.9/sqrt(err), 0.2, 4).
 template <class R>
 int integrate_system(int method, const oracle_net *net, R tnow, R tend, R h, R *y, const R *c,
                      int n, R ABS, R REL, Work<R> &w, oracle_trace *tr,
-                     long long max_attempts = 0, int *status = nullptr) {
+                     long long max_attempts = 0, int *status = nullptr, int controller = 0) {
     std::vector<R> y1(y, y + n), y2(y, y + n);
     int nsteps = 0, unsolved = 0;
     long long attempts = 0;
@@ -487,6 +500,35 @@ int integrate_system(int method, const oracle_net *net, R tnow, R tend, R h, R *
         R tb = tnow + h;
         nsteps += take_step<R>(method, net, tnow, tb, 1, y1.data(), c, n, w);
         nsteps += take_step<R>(method, net, tnow, tb, 2, y2.data(), c, n, w);
+        if (controller == ORACLE_CTRL_ADAPTIVE) {
+            double err = 0.0;
+            for (int i = 0; i < n; i++) {
+                double e = std::fabs((double)y2[i] - (double)y1[i]) /
+                           ((double)ABS + (double)REL * std::fabs((double)y2[i]));
+                if (e > err || e != e || err != err) err = (err != err) ? err : e;
+            }
+            const bool ok = err <= 1.0;
+            record(tr, n, tnow, h, ok ? 0 : 1, y1.data(), y2.data());
+            double fac;
+            if (ok)
+                fac = err > 0.0 ? 0.9 / std::sqrt(err) : 4.0;
+            else
+                fac = (err == err) ? 0.9 / std::sqrt(err) : 0.2;
+            fac = std::fmin(4.0, std::fmax(0.2, fac));
+            R hh = h;
+            h = (R)((double)hh * fac);
+            if (ok) {
+                for (int i = 0; i < n; i++) y[i] = 2 * y2[i] - y1[i];
+                tnow += hh;
+            }
+            std::memcpy(y1.data(), y, sizeof(R) * n);
+            std::memcpy(y2.data(), y, sizeof(R) * n);
+            if (h == (R)0 && tnow < tend) {
+                if (status) *status = ORACLE_STATUS_STALLED;
+                break;
+            }
+            continue;
+        }
         int flag = check_error<R>(method, y1.data(), y2.data(), n, ABS, REL);
         record(tr, n, tnow, h, flag, y1.data(), y2.data());
         if (flag && unsolved < 10) {
@@ -523,14 +565,14 @@ int integrate_system(int method, const oracle_net *net, R tnow, R tend, R h, R *
 template <class R>
 int gold_integrate(int method, const oracle_net *net, R tnow, R tend, int nfix, const R *C,
                    int cstride, R *Y, int ns, int n, R ABS, R REL, int32_t *steps_out,
-                   long long max_attempts = 0, int32_t *status_out = nullptr) {
+                   long long max_attempts = 0, int32_t *status_out = nullptr, int controller = 0) {
     Work<R> w(n);
     int total = 0;
     for (int s = 0; s < ns; s++) {
         int st = 0;
         int k = integrate_system<R>(method, net, tnow, tend, (tend - tnow) / nfix, Y + (size_t)n * s,
                                     C + (size_t)cstride * s, n, ABS, REL, w, nullptr,
-                                    max_attempts, &st);
+                                    max_attempts, &st, controller);
         if (steps_out) steps_out[s] = k;
         if (status_out) status_out[s] = st;
         total += k;
@@ -558,25 +600,25 @@ int oracle_gold_integrate_ex(int method, const oracle_net *net, float tnow, floa
                              int n_integration_steps, const float *constants, int const_stride,
                              float *equations, int Nsystems, int Neqn, float ABSOLUTE,
                              float RELATIVE, int32_t *steps_out, long long max_attempts,
-                             int32_t *status_out) {
+                             int32_t *status_out, int controller) {
     return gold_integrate<float>(method, net, tnow, tend, n_integration_steps, constants,
                                  const_stride, equations, Nsystems, Neqn, ABSOLUTE, RELATIVE,
-                                 steps_out, max_attempts, status_out);
+                                 steps_out, max_attempts, status_out, controller);
 }
 
 int oracle_gold_integrate_f64(int method, const oracle_net *net, double tnow, double tend,
                               int n_integration_steps, const double *constants, int const_stride,
                               double *equations, int Nsystems, int Neqn, double ABSOLUTE,
-                              double RELATIVE, int32_t *steps_out) {
+                              double RELATIVE, int32_t *steps_out, int controller) {
     return gold_integrate<double>(method, net, tnow, tend, n_integration_steps, constants,
                                   const_stride, equations, Nsystems, Neqn, ABSOLUTE, RELATIVE,
-                                  steps_out);
+                                  steps_out, 0, nullptr, controller);
 }
 
 int oracle_gold_integrate_mt(int method, const oracle_net *net, float tnow, float tend,
                              int n_integration_steps, const float *constants, int const_stride,
                              float *equations, int Nsystems, int Neqn, float ABSOLUTE,
-                             float RELATIVE, int32_t *steps_out, int nthreads) {
+                             float RELATIVE, int32_t *steps_out, int nthreads, int controller) {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > Nsystems) nthreads = Nsystems > 0 ? Nsystems : 1;
     std::vector<int> part(nthreads, 0);
@@ -588,7 +630,8 @@ int oracle_gold_integrate_mt(int method, const oracle_net *net, float tnow, floa
             part[t] = gold_integrate<float>(method, net, tnow, tend, n_integration_steps,
                                             constants + (size_t)const_stride * lo, const_stride,
                                             equations + (size_t)Neqn * lo, hi - lo, Neqn, ABSOLUTE,
-                                            RELATIVE, steps_out ? steps_out + lo : nullptr);
+                                            RELATIVE, steps_out ? steps_out + lo : nullptr, 0, nullptr,
+                                            controller);
         });
     }
     int total = 0;

@@ -49,6 +49,10 @@ enum oracle_network {
 
 enum oracle_method { ORACLE_SIE = 0, ORACLE_RK2 = 1 };
 
+/* Timestep controller: REFERENCE restates common_gold.c:116-148; ADAPTIVE is
+ * the library's error-norm controller (an extension, see wind_oracle.cpp). */
+enum oracle_controller { ORACLE_CTRL_REFERENCE = 0, ORACLE_CTRL_ADAPTIVE = 1 };
+
 /* Per-system termination status (the reference has no such guards). */
 enum oracle_status {
     ORACLE_STATUS_OK = 0,
@@ -95,7 +99,7 @@ int oracle_gold_integrate_ex(int method, const oracle_net *net,
                              float *equations, int Nsystems, int Neqn,
                              float ABSOLUTE, float RELATIVE,
                              int32_t *steps_out, long long max_attempts,
-                             int32_t *status_out);
+                             int32_t *status_out, int controller);
 
 /* Same algorithm in double (state, time and tolerances), used for
  * FP64 reference integrations. */
@@ -104,7 +108,7 @@ int oracle_gold_integrate_f64(int method, const oracle_net *net,
                               const double *constants, int const_stride,
                               double *equations, int Nsystems, int Neqn,
                               double ABSOLUTE, double RELATIVE,
-                              int32_t *steps_out);
+                              int32_t *steps_out, int controller);
 
 /* One system with an attempt trace (restates integrateSystem,
  * common_gold.c:48-153). */
@@ -122,7 +126,7 @@ int oracle_gold_integrate_mt(int method, const oracle_net *net,
                              const float *constants, int const_stride,
                              float *equations, int Nsystems, int Neqn,
                              float ABSOLUTE, float RELATIVE,
-                             int32_t *steps_out, int nthreads);
+                             int32_t *steps_out, int nthreads, int controller);
 
 /* Stage-level entry points for known-answer tests. */
 void oracle_gje_invert(float *matrix, float *inverse, int N);              /* gje_gold.c:136-144 */

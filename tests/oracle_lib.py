@@ -38,9 +38,9 @@ def lib():
         L = C.CDLL(ORACLE_SO)
         vp, i, f, d = C.c_void_p, C.c_int, C.c_float, C.c_double
         L.oracle_gold_integrate.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp]
-        L.oracle_gold_integrate_ex.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, C.c_longlong, vp]
-        L.oracle_gold_integrate_mt.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, i]
-        L.oracle_gold_integrate_f64.argtypes = [i, vp, d, d, i, vp, i, vp, i, i, d, d, vp]
+        L.oracle_gold_integrate_ex.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, C.c_longlong, vp, i]
+        L.oracle_gold_integrate_mt.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, i, i]
+        L.oracle_gold_integrate_f64.argtypes = [i, vp, d, d, i, vp, i, vp, i, i, d, d, vp, i]
         L.oracle_integrate_system_traced.argtypes = [i, vp, f, f, f, vp, vp, i, f, f, vp, C.c_longlong, vp]
         L.oracle_gje_invert.argtypes = [vp, vp, i]
         L.oracle_gje_invert.restype = None
@@ -103,7 +103,7 @@ def _f32(a):
 
 
 def gold_integrate(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL,
-                   const_stride=None, max_attempts=0, threads=1):
+                   const_stride=None, max_attempts=0, threads=1, controller=0):
     """Returns (total, steps[nsys], status[nsys]); `equations` updated in place."""
     L = lib()
     stride = len(constants) // max(nsys, 1) if const_stride is None else const_stride
@@ -112,20 +112,22 @@ def gold_integrate(method, net, tnow, tend, nfix, constants, equations, nsys, AB
     if threads > 1:
         tot = L.oracle_gold_integrate_mt(method, net.ref, tnow, tend, nfix, constants.ctypes.data, stride,
                                          equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data,
-                                         threads)
+                                         threads, controller)
     else:
         tot = L.oracle_gold_integrate_ex(method, net.ref, tnow, tend, nfix, constants.ctypes.data, stride,
                                          equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data,
-                                         max_attempts, status.ctypes.data)
+                                         max_attempts, status.ctypes.data, controller)
     return tot, steps, status
 
 
-def gold_integrate_f64(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL, const_stride=None):
+def gold_integrate_f64(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL, const_stride=None,
+                       controller=0):
     L = lib()
     stride = len(constants) // max(nsys, 1) if const_stride is None else const_stride
     steps = np.zeros(nsys, np.int32)
     tot = L.oracle_gold_integrate_f64(method, net.ref, tnow, tend, nfix, constants.ctypes.data, stride,
-                                      equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data)
+                                      equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data,
+                                      controller)
     return tot, steps
 
 

@@ -35,19 +35,22 @@ def first_diff(a, b):
     return None if len(bad) == 0 else (int(bad[0]), float(a[bad[0]]), float(b[bad[0]]), len(bad))
 
 
-def run_gpu(net, tnow, tend, nfix, c, y, nsys, tol, method, kernel, mode=api.MODE_EXACT, max_attempts=0):
+def run_gpu(net, tnow, tend, nfix, c, y, nsys, tol, method, kernel, mode=api.MODE_EXACT, max_attempts=0,
+            controller=api.CTRL_REFERENCE):
     y = y.copy()
     steps = np.zeros(nsys, np.int32)
     status = np.zeros(nsys, np.int32)
     total, _ = api.integrate(net, tnow, tend, nfix, c, y, nsys, tol, tol, method=method, mode=mode,
-                             kernel=kernel, steps_out=steps, status_out=status, max_attempts=max_attempts)
+                             kernel=kernel, steps_out=steps, status_out=status, max_attempts=max_attempts,
+                             controller=controller)
     return y, steps, status, total
 
 
-def run_oracle(onet, tnow, tend, nfix, c, y, nsys, tol, method, stride, max_attempts=0):
+def run_oracle(onet, tnow, tend, nfix, c, y, nsys, tol, method, stride, max_attempts=0, controller=0):
     y = y.copy()
     total, steps, status = O.gold_integrate(method, onet, tnow, tend, nfix, c, y, nsys, tol, tol,
-                                            const_stride=stride, max_attempts=max_attempts)
+                                            const_stride=stride, max_attempts=max_attempts,
+                                            controller=controller)
     return y, steps, status, total
 
 
@@ -140,3 +143,60 @@ def test_legacy_symbol_reference_fixture():
             n = fn(0.0, float(tend), 1, KATZ_FIXTURE_C.ctypes.data, eq.ctypes.data, 4, 5, 5e-3, 5e-3)
             assert n == int(G[f"fixture_t{tend}_{name}_n"]), (name, tend, n)
             assert same(eq, G[f"fixture_t{tend}_{name}_eq"]), (name, tend)
+
+
+CTRLS = [api.CTRL_ADAPTIVE]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_adaptive_controller_exact(kernel, method):
+    """Exact arithmetic with the adaptive controller against the oracle's
+    restatement of the same controller: NR_test, Katz96 and hhe6."""
+    y0, c, _ = W.nr_test_systems(512, seed=4)
+    g = run_gpu(api.Network.builtin(api.NET_NR_TEST), 0.0, 1.0, 1000, c, y0, 512, 1e-4, method, kernel,
+                controller=api.CTRL_ADAPTIVE, max_attempts=5000)
+    r = run_oracle(O.nr_test(), 0.0, 1.0, 1000, c, y0, 512, 1e-4, method, 4, controller=1, max_attempts=5000)
+    check(g, r, f"nr_test adaptive m{method} k{kernel}")
+    y0, c = katz_inputs(200, seed=12)
+    g = run_gpu(api.Network.builtin(api.NET_KATZ96), 0.0, 50.0, 1, c, y0, 200, 1e-4, method, kernel,
+                controller=api.CTRL_ADAPTIVE, max_attempts=3000)
+    r = run_oracle(O.katz96(), 0.0, 50.0, 1, c, y0, 200, 1e-4, method, 10, controller=1, max_attempts=3000)
+    check(g, r, f"katz96 adaptive m{method} k{kernel}")
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(256, seed=6)
+    g = run_gpu(api.Network.builtin(api.NET_HHE6, p), 0.0, 1.0e4, 1, nH, y0, 256, 1e-4, method, kernel,
+                controller=api.CTRL_ADAPTIVE, max_attempts=3000)
+    r = run_oracle(O.hhe6(p), 0.0, 1.0e4, 1, nH, y0, 256, 1e-4, method, 1, controller=1, max_attempts=3000)
+    check(g, r, f"hhe6 adaptive m{method} k{kernel}")
+
+
+@pytest.mark.parametrize("N", [10, 32])
+def test_mass_action_adaptive_exact(N):
+    rx, k = W.mass_action_network(N, seed=300 + N)
+    y0, sc = W.mass_action_systems(32, N, seed=400 + N, scale_range=(0.3, 3.0))
+    g = run_gpu(api.Network.mass_action(N, rx, k), 0.0, W.MASS_ACTION_TEND, 1, sc, y0, 32, 1e-4, api.SIE,
+                api.KERNEL_AUTO, controller=api.CTRL_ADAPTIVE, max_attempts=3000)
+    r = run_oracle(O.mass_action(N, rx, k), 0.0, W.MASS_ACTION_TEND, 1, sc, y0, 32, 1e-4, api.SIE, 1,
+                   controller=1, max_attempts=3000)
+    check(g, r, f"mass_action adaptive N{N}")
+
+
+def test_fast_mode_accuracy_vs_fp64():
+    """Fast arithmetic + adaptive controller at tol 1e-5 on the 512 sampled
+    systems of the 2^20-system hhe6 workload against FP64 Radau solutions:
+    final-state relative error (floor 1e-3) <= 1e-3 for at least 95% of them
+    (the oracle's float32 run of the same controller reaches 97.1%)."""
+    T = np.load(os.path.join(ROOT, "tests", "golden", "truth_hhe6.npz"))
+    idx, yt = T["idx"], T["y"]
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(int(T["nsys"]), seed=int(T["seed"]))
+    Y = y0.reshape(-1, 6)[idx].reshape(-1).copy()
+    c = nH[idx].copy()
+    g = run_gpu(api.Network.builtin(api.NET_HHE6, p), 0.0, float(T["tend"]), 1, c, Y, len(idx), 1e-5, api.SIE,
+                api.KERNEL_LANE, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE, max_attempts=100000)
+    e = np.max(np.abs(g[0].reshape(-1, 6) - yt) / np.maximum(np.abs(yt), 1e-3), axis=1)
+    e = np.where(np.isnan(e), np.inf, e)
+    print("fast adaptive: frac<=1e-3", np.mean(e <= 1e-3), "median", np.median(e), "max", e.max())
+    assert np.mean(e <= 1e-3) >= 0.95
+    assert (g[2] == 0).all()

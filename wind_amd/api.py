@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libwind_amd.so")
 SIE, RK2 = 0, 1
 MODE_EXACT, MODE_FAST = 0, 1
 KERNEL_AUTO, KERNEL_BLOCK, KERNEL_LANE = 0, 1, 2
+CTRL_REFERENCE, CTRL_ADAPTIVE = 0, 1
 NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION = 0, 1, 2, 3
 STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 
@@ -33,7 +34,7 @@ class WindOptions(C.Structure):
         ("method", C.c_int), ("mode", C.c_int), ("kernel", C.c_int), ("const_stride", C.c_int),
         ("device_pointers", C.c_int), ("max_attempts", C.c_int64),
         ("steps_out", C.c_void_p), ("status_out", C.c_void_p), ("total_steps", C.c_void_p),
-        ("stream", C.c_void_p), ("kernel_ms", C.c_void_p),
+        ("stream", C.c_void_p), ("kernel_ms", C.c_void_p), ("controller", C.c_int),
     ]
 
 
@@ -132,13 +133,14 @@ class Network:
 def integrate(net: Network, tnow, tend, n_integration_steps, constants, equations, nsys,
               absolute, relative, method=SIE, mode=MODE_EXACT, kernel=KERNEL_AUTO,
               const_stride=0, device_pointers=False, max_attempts=0, steps_out=None,
-              status_out=None, stream=None, want_kernel_ms=False):
+              status_out=None, stream=None, want_kernel_ms=False, controller=CTRL_REFERENCE):
     """Integrates `nsys` systems in place.  Returns (total_substeps, kernel_ms)."""
     lib = load()
     o = WindOptions()
     lib.wind_options_init(C.byref(o))
     o.method, o.mode, o.kernel, o.const_stride = method, mode, kernel, const_stride
     o.device_pointers = 1 if device_pointers else 0
+    o.controller = controller
     o.max_attempts = max_attempts
     total = C.c_int64(0)
     kms = C.c_float(0.0)

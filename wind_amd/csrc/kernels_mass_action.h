@@ -166,6 +166,7 @@ __global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassA
     L.M = mats;
     L.Inv = mats + N * N;
     __shared__ int s_flag;
+    __shared__ double s_err[4];
     unsigned long long steps = 0;
     for (int sid = blockIdx.x; sid < a.nsys; sid += gridDim.x) {
         for (int i = t; i < N; i += T) L.y[i] = a.equations[(size_t)sid * N + i];
@@ -203,29 +204,31 @@ __global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassA
             if (t == 0) s_flag = 0;
             __syncthreads();
             bool fail = false;
-            for (int i = t; i < N; i += T)
-                fail = fail || species_fails<METHOD>(L.y1[i], L.yw[i], a.ABS, a.REL);
-            if (fail) s_flag = 1;
-            __syncthreads();
-            const bool flag = s_flag != 0;
-            attempts++;
-            if (flag && unsolved < 10) {
-                h = hh * 0.5f;
-                unsolved++;
-            } else {
+            double err = 0.0;
+            if (a.controller == WIND_CTRL_REFERENCE) {
                 for (int i = t; i < N; i += T)
-                    L.y[i] = (METHOD == WIND_RK2) ? 2.0f * L.yw[i] - L.y1[i] : L.yw[i];
-                const float tprev = tnow;
-                tnow = tnow + hh;
-                h = (METHOD == WIND_RK2) ? hh * 2.0f : a.tend - tnow;
-                unsolved = 0;
-                if (tnow == tprev && (METHOD == WIND_SIE || h == 0.0f)) {
-                    status = WIND_STATUS_STALLED;
-                    __syncthreads();
-                    break;
-                }
+                    fail = fail || species_fails<METHOD>(L.y1[i], L.yw[i], a.ABS, a.REL);
+                if (fail) s_flag = 1;
+            } else {
+                for (int i = t; i < N; i += T) err = err_max(err, err_term(L.y1[i], L.yw[i], a.ABS, a.REL));
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) err = err_max(err, __shfl_xor(err, off));
+                if (__lane_id() == 0) s_err[t >> 6] = err;
             }
             __syncthreads();
+            const bool flag = s_flag != 0;
+            if (a.controller != WIND_CTRL_REFERENCE) {
+                err = s_err[0];
+                for (int wv = 1; wv < (T + 63) / 64; wv++) err = err_max(err, s_err[wv]);
+            }
+            attempts++;
+            const bool acc = control_step<METHOD>(a.controller, a.tend, hh, flag, err, tnow, h, unsolved, status);
+            if (acc) {
+                const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
+                for (int i = t; i < N; i += T) L.y[i] = extrap ? 2.0f * L.yw[i] - L.y1[i] : L.yw[i];
+            }
+            __syncthreads();
+            if (status != WIND_STATUS_OK) break;
             if (attempts >= a.max_attempts && tnow < a.tend) {
                 status = WIND_STATUS_EXHAUSTED;
                 break;

@@ -22,8 +22,6 @@
 
 namespace wind {
 
-constexpr int kRefineCap = 10;  // common_gold.c:121 (both SIE and RK2)
-
 // ---------------------------------------------------------------------------
 // Exact-mode solve: the reference's explicit Gauss-Jordan inverse of the
 // stored matrix followed by its transposed mat-vec (gje_gold.c:56-144,
@@ -145,8 +143,8 @@ struct LaneSys {
     int status;
 };
 
-// One step-doubling attempt (common_gold.c:77-148).  Returns true when the
-// system is finished.
+// One step-doubling attempt (common_gold.c:77-148) followed by the timestep
+// control of a.controller.  Returns true when the system is finished.
 template <class Net, int METHOD, class M>
 __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArgs &a, const float *c,
                                LaneSys<Net::N> &s) {
@@ -165,24 +163,22 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
     substep<Net, METHOD, M>(p, c, y2, hs2);
     substep<Net, METHOD, M>(p, c, y2, hs2);
     bool flag = false;
+    double err = 0.0;
+    if (a.controller == WIND_CTRL_REFERENCE) {
 #pragma unroll
-    for (int i = 0; i < N; i++) flag = flag || species_fails<METHOD>(y1[i], y2[i], a.ABS, a.REL);
-    s.attempts++;
-    if (flag && s.unsolved < kRefineCap) {
-        s.h = hh * 0.5f;
-        s.unsolved++;
+        for (int i = 0; i < N; i++) flag = flag || species_fails<METHOD>(y1[i], y2[i], a.ABS, a.REL);
     } else {
 #pragma unroll
-        for (int i = 0; i < N; i++) s.y[i] = (METHOD == WIND_RK2) ? 2.0f * y2[i] - y1[i] : y2[i];
-        const float tprev = s.tnow;
-        s.tnow = s.tnow + hh;
-        s.h = (METHOD == WIND_RK2) ? hh * 2.0f : a.tend - s.tnow;
-        s.unsolved = 0;
-        if (s.tnow == tprev && (METHOD == WIND_SIE || s.h == 0.0f)) {
-            s.status = WIND_STATUS_STALLED;
-            return true;
-        }
+        for (int i = 0; i < N; i++) err = err_max(err, err_term(y1[i], y2[i], a.ABS, a.REL));
     }
+    s.attempts++;
+    const float hh_used = hh;
+    if (control_step<METHOD>(a.controller, a.tend, hh_used, flag, err, s.tnow, s.h, s.unsolved, s.status)) {
+        const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
+#pragma unroll
+        for (int i = 0; i < N; i++) s.y[i] = extrap ? 2.0f * y2[i] - y1[i] : y2[i];
+    }
+    if (s.status != WIND_STATUS_OK) return true;
     if (!(s.tnow < a.tend)) return true;
     if (s.attempts >= a.max_attempts) {
         s.status = WIND_STATUS_EXHAUSTED;
@@ -335,24 +331,23 @@ __global__ void __launch_bounds__(64) block_integrate(IntegrateArgs a, typename 
                 if (pass == 0 && t < N) y1[t] = yw[t];
             }
             bool fail = false;
-            if (t < N) fail = species_fails<METHOD>(y1[t], yw[t], a.ABS, a.REL);
+            double err = 0.0;
+            if (a.controller == WIND_CTRL_REFERENCE) {
+                if (t < N) fail = species_fails<METHOD>(y1[t], yw[t], a.ABS, a.REL);
+            } else {
+                if (t < N) err = err_term(y1[t], yw[t], a.ABS, a.REL);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) err = err_max(err, __shfl_xor(err, off));
+            }
             const bool flag = __any(fail);
             attempts++;
-            if (flag && unsolved < kRefineCap) {
-                h = hh * 0.5f;
-                unsolved++;
-            } else {
-                if (t < N) y[t] = (METHOD == WIND_RK2) ? 2.0f * yw[t] - y1[t] : yw[t];
-                const float tprev = tnow;
-                tnow = tnow + hh;
-                h = (METHOD == WIND_RK2) ? hh * 2.0f : a.tend - tnow;
-                unsolved = 0;
-                if (tnow == tprev && (METHOD == WIND_SIE || h == 0.0f)) {
-                    status = WIND_STATUS_STALLED;
-                    break;
-                }
+            const bool acc = control_step<METHOD>(a.controller, a.tend, hh, flag, err, tnow, h, unsolved, status);
+            if (acc && t < N) {
+                const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
+                y[t] = extrap ? 2.0f * yw[t] - y1[t] : yw[t];
             }
             __syncthreads();
+            if (status != WIND_STATUS_OK) break;
             if (attempts >= a.max_attempts && tnow < a.tend) {
                 status = WIND_STATUS_EXHAUSTED;
                 break;

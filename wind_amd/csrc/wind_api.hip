@@ -230,6 +230,8 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
         return fail(WIND_ERR_ARGUMENT, "method must be WIND_SIE or WIND_RK2");
     if (opts.mode != WIND_MODE_EXACT && opts.mode != WIND_MODE_FAST)
         return fail(WIND_ERR_ARGUMENT, "mode must be WIND_MODE_EXACT or WIND_MODE_FAST");
+    if (opts.controller != WIND_CTRL_REFERENCE && opts.controller != WIND_CTRL_ADAPTIVE)
+        return fail(WIND_ERR_ARGUMENT, "controller must be WIND_CTRL_REFERENCE or WIND_CTRL_ADAPTIVE");
     if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
     if (Neqn != net->N)
         return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network (N = " +
@@ -282,6 +284,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     a.h0 = (tend - tnow) / (float)n_integration_steps;  // sie_gold.c:105
     a.ABS = ABSOLUTE;
     a.REL = RELATIVE;
+    a.controller = opts.controller;
     a.constants = d_const;
     a.equations = d_eq;
     a.nsys = Nsystems;

@@ -134,8 +134,10 @@ struct NetKatz96 {
 // radiative and dielectronic He+ recombination; 6 He++ recombination) and
 // photoionisation rates Gamma_0..2 of H0, He0, He+.  The energy equation is
 // photoheating minus collisional-ionisation, recombination and free-free
-// cooling; T = 1e4 e / (1.5 mu), floored at 10 K.  The Jacobian is the
-// exact derivative of this RHS away from the temperature floor.
+// cooling.  T_raw = 1e4 e / (1.5 mu); the rates use T = max(T_raw, 10 K) and
+// the cooling is scaled by phi = min(1, T_raw / 10 K), so it vanishes as
+// e -> 0.  The Jacobian is the exact derivative of this RHS (one-sided at
+// the floor).
 struct NetHHe6 {
     static constexpr int N = 6;
     static constexpr int NCONST = 1;
@@ -148,8 +150,8 @@ struct NetHHe6 {
     static constexpr float LFF = 1.0285e-15f, RECC = 0.75f;
 
     struct State {
-        float xe, mu, T, lnT, invT, nHyr, sqT;
-        float Rj[7], gj[7], G[3], dT[6];
+        float xe, mu, T, lnT, invT, nHyr, sqT, phi;
+        float Rj[7], gj[7], G[3], dT[6], dphi[6];
     };
 
     template <class M>
@@ -158,9 +160,10 @@ struct NetHHe6 {
         s.xe = (y[1] + y[3]) + (y[4] + y[4]);
         s.mu = (((y[0] + y[1]) + (y[2] + y[3])) + y[4]) + s.xe;
         float dTde = TUNIT / (1.5f * s.mu);
-        float T = y[5] * dTde;
-        bool clamped = !(T > TFLOOR);
-        if (clamped) T = TFLOOR;
+        float Traw = y[5] * dTde;
+        bool clamped = !(Traw > TFLOOR);
+        float T = clamped ? TFLOOR : Traw;
+        s.phi = clamped ? Traw * 0.1f : 1.0f;
         s.T = T;
         s.lnT = M::lg(T);
         s.invT = 1.0f / T;
@@ -174,15 +177,16 @@ struct NetHHe6 {
         for (int j = 0; j < 3; j++) s.G[j] = p.Gamma[j] * YR;
         s.sqT = M::ex(0.5f * s.lnT);
         if (want_derivs) {
-            if (clamped) {
+            float Tmu = Traw / s.mu;
+            const float dmu[5] = {1.0f, 2.0f, 1.0f, 2.0f, 3.0f};
+            float dTraw[6];
 #pragma unroll
-                for (int k = 0; k < 6; k++) s.dT[k] = 0.0f;
-            } else {
-                float Tmu = T / s.mu;
-                const float dmu[5] = {1.0f, 2.0f, 1.0f, 2.0f, 3.0f};
+            for (int k = 0; k < 5; k++) dTraw[k] = -(Tmu * dmu[k]);
+            dTraw[5] = dTde;
 #pragma unroll
-                for (int k = 0; k < 5; k++) s.dT[k] = -(Tmu * dmu[k]);
-                s.dT[5] = dTde;
+            for (int k = 0; k < 6; k++) {
+                s.dT[k] = clamped ? 0.0f : dTraw[k];
+                s.dphi[k] = clamped ? dTraw[k] * 0.1f : 0.0f;
             }
         }
     }
@@ -211,7 +215,7 @@ struct NetHHe6 {
         float crec = ((w3 + w4) + w5) * (RECC * T4);
         float ion = (y[1] + y[3]) + 4.0f * y[4];
         float cff = ((LFF * s.nHyr) * s.sqT) * (xe * ion);
-        f[5] = heat - ((cci + crec) + cff);
+        f[5] = heat - ((cci + crec) + cff) * s.phi;
     }
 
     // d/dy_k of w = (R xe + G) y_s.
@@ -255,6 +259,9 @@ struct NetHHe6 {
         float LNs = LN * s.sqT;
         float xion = xe * ion;
         float dsqc = (0.5f * s.sqT) * s.invT;
+        float cci = (((s.Rj[0] * xe) * y[0]) * CHI0 + ((s.Rj[1] * xe) * y[2]) * CHI1) +
+                    ((s.Rj[2] * xe) * y[3]) * CHI2;
+        float cool = (cci + W * recT) + LNs * xion;
         const float dxe[6] = {0.0f, 1.0f, 0.0f, 1.0f, 2.0f, 0.0f};
         const float dion[6] = {0.0f, 1.0f, 0.0f, 1.0f, 4.0f, 0.0f};
 #pragma unroll
@@ -272,7 +279,7 @@ struct NetHHe6 {
             float dcci = (dc0[k] * CHI0 + dc1[k] * CHI1) + dc2[k] * CHI2;
             float dcrec = ((dw3[k] + dw4[k]) + dw5[k]) * recT + W * (RECC * (s.dT[k] * 1.0e-4f));
             float dcff = (LN * (dsqc * s.dT[k])) * xion + LNs * (dxe[k] * ion + xe * dion[k]);
-            col[5] = heat - ((dcci + dcrec) + dcff);
+            col[5] = heat - (((dcci + dcrec) + dcff) * s.phi + cool * s.dphi[k]);
         }
     }
 };
@@ -296,5 +303,57 @@ __device__ inline bool species_fails(float y1, float y2, float ABS, float REL) {
     double rel = fabs((double)d / den);
     return rel >= (double)REL && fabsf(y1) > ABS && fabsf(y2) > ABS;
 }
+
+// ---------------------------------------------------------------------------
+// Timestep control after one step-doubling attempt of size hh.
+//
+// REFERENCE restates common_gold.c:121-148: on failure halve (at most
+// kRefineCap times in a row, then accept anyway); on acceptance advance tnow
+// and retry the whole remaining interval (SIE) or double the step (RK2).
+// ADAPTIVE accepts iff the double-precision error norm is <= 1 (a NaN norm
+// fails) and rescales the step by clamp(0.9/sqrt(err), 0.2, 4).
+// Both set status to STALLED where the next attempt could never advance:
+// an accepted step that left tnow unchanged with nothing left to grow it
+// (reference), or a step that underflowed to zero.
+// Returns true when the attempt is accepted.
+// ---------------------------------------------------------------------------
+constexpr int kRefineCap = 10;  // common_gold.c:121
+
+template <int METHOD>
+__device__ inline bool control_step(int controller, float tend, float hh, bool ref_flag, double err,
+                                    float &tnow, float &h, int &unsolved, int &status) {
+    if (controller == WIND_CTRL_REFERENCE) {
+        if (ref_flag && unsolved < kRefineCap) {
+            h = hh * 0.5f;
+            unsolved++;
+            return false;
+        }
+        const float tprev = tnow;
+        tnow = tnow + hh;
+        h = (METHOD == WIND_RK2) ? hh * 2.0f : tend - tnow;
+        unsolved = 0;
+        if (tnow == tprev && (METHOD == WIND_SIE || h == 0.0f)) status = WIND_STATUS_STALLED;
+        return true;
+    }
+    const bool ok = err <= 1.0;
+    double fac;
+    if (ok)
+        fac = err > 0.0 ? 0.9 / sqrt(err) : 4.0;
+    else
+        fac = (err == err) ? 0.9 / sqrt(err) : 0.2;
+    fac = fmin(4.0, fmax(0.2, fac));
+    h = (float)((double)hh * fac);
+    if (ok) tnow = tnow + hh;
+    if (h == 0.0f && tnow < tend) status = WIND_STATUS_STALLED;
+    return ok;
+}
+
+// One species' term of the adaptive error norm, |y2 - y1| / (ABS + REL |y2|).
+__device__ inline double err_term(float y1, float y2, float ABS, float REL) {
+    return fabs((double)y2 - (double)y1) / ((double)ABS + (double)REL * fabs((double)y2));
+}
+
+// max() that keeps a NaN once one has been seen (a NaN norm must fail).
+__device__ inline double err_max(double a, double b) { return (b > a || b != b || a != a) ? (a != a ? a : b) : a; }
 
 }  // namespace wind

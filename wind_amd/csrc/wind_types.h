@@ -12,6 +12,7 @@ namespace wind {
 struct IntegrateArgs {
     float tnow, tend, h0;          // h0 = (tend - tnow) / n_integration_steps
     float ABS, REL;
+    int controller;                // wind_controller
     const float *constants;
     float *equations;
     int32_t nsys;
