@@ -38,7 +38,8 @@ def test_gje_invert(N):
 @pytest.mark.parametrize("N", [2, 5, 6, 32])
 def test_matvec(N):
     v = G[f"matvec_N{N}_v"].copy()
-    O.lib().oracle_matvec(G[f"matvec_N{N}_M"].ctypes.data, v.ctypes.data, N)
+    M = G[f"matvec_N{N}_M"]
+    O.lib().oracle_matvec(M.ctypes.data, v.ctypes.data, N)
     assert same_bits(v, G[f"matvec_N{N}_out"])
 
 
@@ -46,8 +47,10 @@ def test_matvec(N):
 def test_check_error_table(method, name):
     y1, y2 = G["check_y1"], G["check_y2"]
     ABS, REL = (float(x) for x in G["check_abs_rel"])
-    flags = [O.lib().oracle_check_error(method, y1[i:i + 1].ctypes.data, y2[i:i + 1].ctypes.data, 1, ABS, REL)
-             for i in range(len(y1))]
+    flags = []
+    for i in range(len(y1)):
+        a, b = y1[i:i + 1], y2[i:i + 1]
+        flags.append(O.lib().oracle_check_error(method, a.ctypes.data, b.ctypes.data, 1, ABS, REL))
     assert np.array_equal(np.array(flags), G[f"check_{name}_flags"])
     assert O.lib().oracle_check_error(method, y1.ctypes.data, y2.ctypes.data, len(y1), ABS, REL) == int(
         G[f"check_{name}_all"])
@@ -71,9 +74,11 @@ def test_katz96_network():
 def test_take_step(method, name, nsub):
     net = O.katz96()
     out = G["take_y"].copy()
+    cs = G["take_c"]
+    hs = G["take_h"]
     for i in range(len(out)):
-        O.lib().oracle_take_step(method, net.ref, 0.0, float(G["take_h"][i]), nsub, out[i].ctypes.data,
-                                 G["take_c"][i].ctypes.data, 5)
+        row, c = out[i], cs[i]
+        O.lib().oracle_take_step(method, net.ref, 0.0, float(hs[i]), nsub, row.ctypes.data, c.ctypes.data, 5)
     assert same_bits(out, G[f"take_{name}_n{nsub}"])
 
 
