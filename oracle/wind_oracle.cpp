@@ -580,9 +580,63 @@ int gold_integrate(int method, const oracle_net *net, R tnow, R tend, int nfix, 
     return total;
 }
 
+// Lockstep SIE, restating cuda/SIE_host/harness_kernel.cu:248-425 (errorLoop):
+// one timestep shared by all systems; each attempt takes the full step and the
+// two half steps (solveSystem with n = 1 and 2, :296-332) for every system;
+// the lockstep error test (vector_kernels.cu:45-70: strict '>', relative test
+// gated on y1 > ABS and y2 > ABS without absolute values) failing for any
+// system halves the step for all; acceptance copies the half-step state and
+// doubles the step; the loop gives up after 9 consecutive failures.  The
+// sub-step arithmetic is c_baseline's (take_step above).  Returns
+// 3 * attempts * Nsystems (:295, :424).
+long long lockstep(const oracle_net *net, float tnow, float tend, int nfix, const float *C, int cstride,
+                   float *Y, int ns, int n, float ABS, float REL) {
+    Work<float> w(n);
+    std::vector<float> Y2((size_t)ns * n), y1(n);
+    long long nsteps = 0;
+    int unsolved = 0;
+    float timestep = (tend - tnow) / nfix;
+    while (tnow < tend && unsolved < 9) {
+        timestep = (float)std::fmin((double)timestep, (double)(tend - tnow));
+        nsteps += 3;
+        int flag = 0;
+        for (int s = 0; s < ns; s++) {
+            const float *c = C + (size_t)cstride * s;
+            float *y2 = Y2.data() + (size_t)n * s;
+            std::memcpy(y1.data(), Y + (size_t)n * s, sizeof(float) * n);
+            std::memcpy(y2, Y + (size_t)n * s, sizeof(float) * n);
+            take_step<float>(ORACLE_SIE, net, tnow, tnow + timestep, 1, y1.data(), c, n, w);
+            take_step<float>(ORACLE_SIE, net, tnow, tnow + timestep, 2, y2, c, n, w);
+            for (int i = 0; i < n; i++) {
+                float d = y1[i] - y2[i];
+                if (std::fabs(d) > ABS) flag = 1;
+                if (std::fabs((double)d / ((double)y2[i] + 1e-12)) > (double)REL && y1[i] > ABS && y2[i] > ABS)
+                    flag = 1;
+            }
+        }
+        if (flag) {
+            unsolved++;
+            timestep /= 2;
+        } else {
+            unsolved = 0;
+            std::memcpy(Y, Y2.data(), sizeof(float) * (size_t)ns * n);
+            tnow += timestep;
+            timestep *= 2;
+        }
+    }
+    return nsteps * ns;
+}
+
 }  // namespace
 
 extern "C" {
+
+long long oracle_lockstep(const oracle_net *net, float tnow, float tend, int n_integration_steps,
+                          const float *constants, int const_stride, float *equations, int Nsystems,
+                          int Neqn, float ABSOLUTE, float RELATIVE) {
+    return lockstep(net, tnow, tend, n_integration_steps, constants, const_stride, equations, Nsystems,
+                    Neqn, ABSOLUTE, RELATIVE);
+}
 
 float oracle_wexpf(float x) { return wexpf(x); }
 float oracle_wlogf(float x) { return wlogf(x); }

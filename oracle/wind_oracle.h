@@ -128,6 +128,12 @@ int oracle_gold_integrate_mt(int method, const oracle_net *net,
                              float ABSOLUTE, float RELATIVE,
                              int32_t *steps_out, int nthreads, int controller);
 
+/* Lockstep SIE with the semantics of cuda/SIE_host/harness_kernel.cu:248-425
+ * and c_baseline sub-step arithmetic.  Returns 3 * attempts * Nsystems. */
+long long oracle_lockstep(const oracle_net *net, float tnow, float tend, int n_integration_steps,
+                          const float *constants, int const_stride, float *equations, int Nsystems,
+                          int Neqn, float ABSOLUTE, float RELATIVE);
+
 /* Stage-level entry points for known-answer tests. */
 void oracle_gje_invert(float *matrix, float *inverse, int N);              /* gje_gold.c:136-144 */
 void oracle_matvec(const float *matrix, float *vector, int N);            /* matrix_ops.c:6-27 */

@@ -56,6 +56,8 @@ def lib():
         L.oracle_rhs_f64.restype = None
         L.oracle_jac_f64.argtypes = [vp, vp, vp, vp]
         L.oracle_jac_f64.restype = None
+        L.oracle_lockstep.argtypes = [vp, f, f, i, vp, i, vp, i, i, f, f]
+        L.oracle_lockstep.restype = C.c_longlong
         L.oracle_wexpf.argtypes = [f]
         L.oracle_wexpf.restype = f
         L.oracle_wlogf.argtypes = [f]
@@ -118,6 +120,12 @@ def gold_integrate(method, net, tnow, tend, nfix, constants, equations, nsys, AB
                                          equations.ctypes.data, nsys, net.N, ABS, REL, steps.ctypes.data,
                                          max_attempts, status.ctypes.data, controller)
     return tot, steps, status
+
+
+def lockstep(net, tnow, tend, nfix, constants, equations, nsys, ABS, REL, const_stride):
+    """Lockstep SIE (the reference's cudaIntegrateSIE semantics); returns 3*attempts*nsys."""
+    return lib().oracle_lockstep(net.ref, tnow, tend, nfix, constants.ctypes.data, const_stride,
+                                 equations.ctypes.data, nsys, net.N, ABS, REL)
 
 
 def gold_integrate_f64(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL, const_stride=None,

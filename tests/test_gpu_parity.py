@@ -200,3 +200,36 @@ def test_fast_mode_accuracy_vs_fp64():
     print("fast adaptive: frac<=1e-3", np.mean(e <= 1e-3), "median", np.median(e), "max", e.max())
     assert np.mean(e <= 1e-3) >= 0.95
     assert (g[2] == 0).all()
+
+
+@pytest.mark.parametrize("which", ["nr_test", "katz96", "hhe6"])
+def test_lockstep_sie_exact(which):
+    """wind_integrate_lockstep (cudaIntegrateSIE semantics) against the oracle's
+    restatement of the lockstep loop, and the sie_host.so drop-in symbol."""
+    if which == "nr_test":
+        y0, c, _ = W.nr_test_systems(256, seed=21)
+        net, onet, stride, t1, nfix = api.Network.builtin(api.NET_NR_TEST), O.nr_test(), 4, 1.0, 1000
+    elif which == "katz96":
+        y0, c = katz_inputs(200, seed=22)
+        net, onet, stride, t1, nfix = api.Network.builtin(api.NET_KATZ96), O.katz96(), 10, 20.0, 1
+    else:
+        p = W.hhe6_params()
+        y0, c = W.hhe6_systems(128, seed=23)
+        net, onet, stride, t1, nfix = api.Network.builtin(api.NET_HHE6, p), O.hhe6(p), 1, 10.0, 10
+    ns = len(y0) // net.N
+    y = y0.copy()
+    got = api.integrate_lockstep(net, 0.0, t1, nfix, c, y, ns, 1e-3, 1e-3, const_stride=stride)
+    yr = y0.copy()
+    want = O.lockstep(onet, 0.0, t1, nfix, c, yr, ns, 1e-3, 1e-3, stride)
+    print(which, "lockstep substeps", got, want)
+    assert got == want
+    assert same(y, yr), first_diff(y, yr)
+    if which == "nr_test":
+        so = C.CDLL(os.path.join(ROOT, "wind_amd", "lib", "nr_test", "sie_host.so"))
+        getattr(so, "_Z26initializeCublasExternallyv")()
+        fn = getattr(so, "_Z16cudaIntegrateSIEffiPfS_iiff")
+        fn.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                       C.c_float]
+        y = y0.copy()
+        assert fn(0.0, t1, nfix, c.ctypes.data, y.ctypes.data, ns, 2, 1e-3, 1e-3) == want
+        assert same(y, yr)

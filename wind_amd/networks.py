@@ -19,7 +19,7 @@ HHE6_DEFAULT_SEED = 20190101
 MASS_ACTION_K_RANGE = (1e-18, 1e-6)
 MASS_ACTION_Y0_RANGE = (1e-12, 1.0)
 # Integration interval of the mass-action workloads, in the network's time unit.
-MASS_ACTION_TEND = 1.0e7
+MASS_ACTION_TEND = 1.0e9
 HHE6_TEND_YR = 1.0e6
 HE_OVER_H = 0.08
 
