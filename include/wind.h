@@ -73,6 +73,20 @@ enum wind_mode { WIND_MODE_EXACT = 0, WIND_MODE_FAST = 1 };
  * clamp(0.9 / sqrt(err), 0.2, 4). */
 enum wind_controller { WIND_CTRL_REFERENCE = 0, WIND_CTRL_ADAPTIVE = 1 };
 
+/* Linear solver of the fast mode for table-driven (mass-action) networks:
+ * (I - hJ) is factored by a blocked LU in LDS whose trailing updates run on
+ * the matrix cores in the chosen precision (VALU: vector FMAs, no MFMA;
+ * BF16: BF16 inputs with FP32 accumulation; BF16X3: split BF16 operands,
+ * hi*hi + hi*lo + lo*hi).  ir_sweeps iterative-refinement sweeps follow,
+ * with the residual evaluated in double from the sparse Jacobian. */
+enum wind_solver {
+    WIND_SOLVER_AUTO = 0,
+    WIND_SOLVER_VALU = 1,
+    WIND_SOLVER_MFMA_F32 = 2,
+    WIND_SOLVER_MFMA_BF16 = 3,
+    WIND_SOLVER_MFMA_BF16X3 = 4
+};
+
 /* Kernel family.  AUTO picks by network and N. */
 enum wind_kernel {
     WIND_KERNEL_AUTO = 0,
@@ -130,7 +144,16 @@ typedef struct wind_options {
     void *stream;          /* hipStream_t; NULL = the library's stream */
     float *kernel_ms;      /* optional: device time of the integration kernel */
     int controller;        /* wind_controller */
+    int solver;            /* wind_solver (fast mode, mass-action networks) */
+    int ir_sweeps;         /* iterative-refinement sweeps after the solve (0-4) */
+    int flags;             /* WIND_FLAG_* */
 } wind_options;
+
+/* wind_options.flags */
+enum wind_flags {
+    WIND_FLAG_GRAPH = 1,       /* batch: capture the fork/join launch into a hipGraph and replay it */
+    WIND_FLAG_NO_AUTOTUNE = 2  /* batch: use default kernel shapes instead of tuning each bucket */
+};
 
 void wind_options_init(wind_options *opts);
 
@@ -140,6 +163,63 @@ void wind_options_init(wind_options *opts);
 int wind_integrate(const wind_network *net, float tnow, float tend, int n_integration_steps,
                    const float *constants, float *equations, int Nsystems, int Neqn,
                    float ABSOLUTE, float RELATIVE, const wind_options *opts);
+
+/* One bucket of a mixed batch: systems of one network, state resident on
+ * the device (device pointers). */
+typedef struct wind_bucket {
+    const wind_network *net;
+    const float *constants;   /* device: (nsys - 1) * const_stride + nconst floats */
+    float *equations;         /* device: nsys * N floats, integrated in place */
+    int nsys;
+    int const_stride;         /* 0 = network default */
+    int32_t *steps_out;       /* optional device [nsys] */
+    int32_t *status_out;      /* optional device [nsys] */
+} wind_bucket;
+
+/* Integrates every bucket from tnow to tend in one fork/join launch: one
+ * persistent kernel per bucket on its own stream, the device shared between
+ * buckets in proportion to their estimated work, each bucket's kernel shape
+ * (workgroup size, solver precision) chosen by an autotuner that times the
+ * candidates on a sample of the bucket's systems (cached per network and
+ * settings).  opts->method/mode/controller/solver apply to all buckets;
+ * opts->total_steps and opts->kernel_ms report the whole batch. */
+int wind_integrate_batch(const wind_bucket *buckets, int nbuckets, float tnow, float tend,
+                         int n_integration_steps, float ABSOLUTE, float RELATIVE,
+                         const wind_options *opts);
+
+/* ---- Request server (continuous batching) ----
+ * A server owns `nslots` execution slots (streams) and a device arena of
+ * `hbm_budget_bytes`.  Submitted requests are admitted in arrival order when
+ * a slot is free and their footprint (constants + state) fits in the arena,
+ * then copied in, integrated (each running kernel takes 1/nslots of the
+ * device, so admitted requests overlap), and copied out.  A resident block
+ * keeps a cell block's state in the arena across requests. */
+typedef struct wind_server wind_server;
+typedef struct wind_block wind_block;
+
+typedef struct wind_request_desc {
+    const wind_network *net;  /* transient request: network, host constants, host state in/out */
+    const float *constants;
+    int const_stride;         /* 0 = network default */
+    float *equations;         /* host; receives the result (optional for block requests) */
+    int nsys;
+    wind_block *block;        /* non-NULL: integrate this resident block in place */
+    float tnow, tend;
+    int n_integration_steps;
+    float ABSOLUTE, RELATIVE;
+    int method, mode, controller;
+} wind_request_desc;
+
+int wind_server_create(size_t hbm_budget_bytes, int nslots, wind_server **out);
+void wind_server_destroy(wind_server *srv);   /* drains queued and running requests */
+int wind_server_block_create(wind_server *srv, const wind_network *net, const float *constants,
+                             int const_stride, const float *equations, int nsys, wind_block **out);
+int wind_server_block_read(wind_server *srv, wind_block *block, float *equations); /* after its requests */
+void wind_server_block_destroy(wind_server *srv, wind_block *block);
+int wind_server_submit(wind_server *srv, const wind_request_desc *desc, int64_t *request_id);
+/* Blocks until the request is done; latency from submission to results on the host. */
+int wind_server_wait(wind_server *srv, int64_t request_id, double *latency_us, int64_t *total_steps);
+int wind_server_inflight(wind_server *srv, int *running, int *pending, size_t *arena_used);
 
 /* Lockstep SIE with the semantics of the reference's cudaIntegrateSIE
  * (cuda/SIE_host/harness_kernel.cu:248-425): one shared timestep, any failing

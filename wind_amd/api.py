@@ -19,13 +19,17 @@ SIE, RK2 = 0, 1
 MODE_EXACT, MODE_FAST = 0, 1
 KERNEL_AUTO, KERNEL_BLOCK, KERNEL_LANE = 0, 1, 2
 CTRL_REFERENCE, CTRL_ADAPTIVE = 0, 1
+SOLVER_AUTO, SOLVER_VALU, SOLVER_MFMA_F32, SOLVER_MFMA_BF16, SOLVER_MFMA_BF16X3 = 0, 1, 2, 3, 4
+FLAG_GRAPH, FLAG_NO_AUTOTUNE = 1, 2
 NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION = 0, 1, 2, 3
 STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 
 EXPORTED = (
     "wind_network_create_builtin", "wind_network_create_mass_action", "wind_network_destroy",
     "wind_network_size", "wind_options_init", "wind_integrate", "wind_integrate_lockstep",
-    "wind_last_error", "wind_version",
+    "wind_last_error", "wind_version", "wind_integrate_batch", "wind_server_create", "wind_server_destroy",
+    "wind_server_block_create", "wind_server_block_read", "wind_server_block_destroy", "wind_server_submit",
+    "wind_server_wait", "wind_server_inflight",
 )
 
 
@@ -35,7 +39,20 @@ class WindOptions(C.Structure):
         ("device_pointers", C.c_int), ("max_attempts", C.c_int64),
         ("steps_out", C.c_void_p), ("status_out", C.c_void_p), ("total_steps", C.c_void_p),
         ("stream", C.c_void_p), ("kernel_ms", C.c_void_p), ("controller", C.c_int),
+        ("solver", C.c_int), ("ir_sweeps", C.c_int), ("flags", C.c_int),
     ]
+
+
+class WindRequestDesc(C.Structure):
+    _fields_ = [("net", C.c_void_p), ("constants", C.c_void_p), ("const_stride", C.c_int),
+                ("equations", C.c_void_p), ("nsys", C.c_int), ("block", C.c_void_p), ("tnow", C.c_float),
+                ("tend", C.c_float), ("n_integration_steps", C.c_int), ("ABSOLUTE", C.c_float),
+                ("RELATIVE", C.c_float), ("method", C.c_int), ("mode", C.c_int), ("controller", C.c_int)]
+
+
+class WindBucket(C.Structure):
+    _fields_ = [("net", C.c_void_p), ("constants", C.c_void_p), ("equations", C.c_void_p), ("nsys", C.c_int),
+                ("const_stride", C.c_int), ("steps_out", C.c_void_p), ("status_out", C.c_void_p)]
 
 
 class WindError(RuntimeError):
@@ -66,6 +83,20 @@ def load(path: str = LIB_PATH):
     lib.wind_integrate_lockstep.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_void_p,
                                             C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_int, C.POINTER(C.c_int64)]
+    lib.wind_integrate_batch.argtypes = [C.POINTER(WindBucket), C.c_int, C.c_float, C.c_float, C.c_int,
+                                         C.c_float, C.c_float, C.POINTER(WindOptions)]
+    lib.wind_server_create.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]
+    lib.wind_server_destroy.argtypes = [C.c_void_p]
+    lib.wind_server_destroy.restype = None
+    lib.wind_server_block_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                             C.POINTER(C.c_void_p)]
+    lib.wind_server_block_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.wind_server_block_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    lib.wind_server_block_destroy.restype = None
+    lib.wind_server_submit.argtypes = [C.c_void_p, C.POINTER(WindRequestDesc), C.POINTER(C.c_int64)]
+    lib.wind_server_wait.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.wind_server_inflight.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_size_t)]
     lib.wind_last_error.restype = C.c_char_p
     lib.wind_version.restype = C.c_char_p
     _lib = lib
@@ -133,7 +164,8 @@ class Network:
 def integrate(net: Network, tnow, tend, n_integration_steps, constants, equations, nsys,
               absolute, relative, method=SIE, mode=MODE_EXACT, kernel=KERNEL_AUTO,
               const_stride=0, device_pointers=False, max_attempts=0, steps_out=None,
-              status_out=None, stream=None, want_kernel_ms=False, controller=CTRL_REFERENCE):
+              status_out=None, stream=None, want_kernel_ms=False, controller=CTRL_REFERENCE,
+              solver=SOLVER_AUTO, ir_sweeps=0):
     """Integrates `nsys` systems in place.  Returns (total_substeps, kernel_ms)."""
     lib = load()
     o = WindOptions()
@@ -141,6 +173,8 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
     o.method, o.mode, o.kernel, o.const_stride = method, mode, kernel, const_stride
     o.device_pointers = 1 if device_pointers else 0
     o.controller = controller
+    o.solver = solver
+    o.ir_sweeps = ir_sweeps
     o.max_attempts = max_attempts
     total = C.c_int64(0)
     kms = C.c_float(0.0)
@@ -167,3 +201,89 @@ def integrate_lockstep(net: Network, tnow, tend, n_integration_steps, constants,
                                        _ptr(equations), nsys, net.N, absolute, relative, const_stride,
                                        C.byref(res)))
     return res.value
+
+
+def integrate_batch(buckets, tnow, tend, n_integration_steps, absolute, relative, method=SIE, mode=MODE_FAST,
+                    controller=CTRL_ADAPTIVE, solver=SOLVER_AUTO, ir_sweeps=0, graph=False, autotune=True,
+                    stream=None, max_attempts=0):
+    """Integrates a mixed batch.  `buckets` is a list of dicts with keys net (Network),
+    constants, equations (device tensors or pointers), nsys, and optional const_stride,
+    steps_out, status_out.  Returns (total_substeps, batch_ms)."""
+    lib = load()
+    arr = (WindBucket * len(buckets))()
+    for i, b in enumerate(buckets):
+        arr[i] = WindBucket(b["net"].handle, _ptr(b["constants"]), _ptr(b["equations"]), b["nsys"],
+                            b.get("const_stride", 0), _ptr(b.get("steps_out")), _ptr(b.get("status_out")))
+    o = WindOptions()
+    lib.wind_options_init(C.byref(o))
+    o.method, o.mode, o.controller, o.solver, o.ir_sweeps = method, mode, controller, solver, ir_sweeps
+    o.flags = (FLAG_GRAPH if graph else 0) | (0 if autotune else FLAG_NO_AUTOTUNE)
+    o.device_pointers = 1
+    o.max_attempts = max_attempts
+    total = C.c_int64(0)
+    kms = C.c_float(0.0)
+    o.total_steps = C.cast(C.byref(total), C.c_void_p)
+    o.kernel_ms = C.cast(C.byref(kms), C.c_void_p)
+    o.stream = stream
+    _check(lib.wind_integrate_batch(arr, len(buckets), tnow, tend, n_integration_steps, absolute, relative,
+                                    C.byref(o)))
+    return total.value, kms.value
+
+
+class Server:
+    """Continuous-batching request server (wind_server_* in include/wind.h)."""
+
+    def __init__(self, hbm_budget_bytes, nslots=4):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.wind_server_create(int(hbm_budget_bytes), nslots, C.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def block(self, net, constants, equations, nsys, const_stride=0):
+        constants = np.ascontiguousarray(constants, np.float32)
+        equations = np.ascontiguousarray(equations, np.float32)
+        h = C.c_void_p()
+        _check(self.lib.wind_server_block_create(self.handle, net.handle, constants.ctypes.data, const_stride,
+                                                 equations.ctypes.data, nsys, C.byref(h)))
+        return h
+
+    def read_block(self, block, out):
+        _check(self.lib.wind_server_block_read(self.handle, block, out.ctypes.data))
+
+    def submit(self, tnow, tend, absolute, relative, net=None, constants=None, equations=None, nsys=0, block=None,
+               n_integration_steps=1, method=SIE, mode=MODE_FAST, controller=CTRL_ADAPTIVE, const_stride=0):
+        """Queues a request; `equations` (host float32) receives the result.  Returns the id."""
+        d = WindRequestDesc(net.handle if net is not None else None,
+                            None if constants is None else constants.ctypes.data, const_stride,
+                            None if equations is None else equations.ctypes.data, nsys, block, tnow, tend,
+                            n_integration_steps, absolute, relative, method, mode, controller)
+        rid = C.c_int64(0)
+        _check(self.lib.wind_server_submit(self.handle, C.byref(d), C.byref(rid)))
+        self._keep[rid.value] = (constants, equations)
+        return rid.value
+
+    def wait(self, rid):
+        """Returns (latency_us, total_substeps)."""
+        lat = C.c_double(0.0)
+        steps = C.c_int64(0)
+        rc = self.lib.wind_server_wait(self.handle, rid, C.byref(lat), C.byref(steps))
+        self._keep.pop(rid, None)
+        _check(rc)
+        return lat.value, steps.value
+
+    def inflight(self):
+        r, p, u = C.c_int(0), C.c_int(0), C.c_size_t(0)
+        _check(self.lib.wind_server_inflight(self.handle, C.byref(r), C.byref(p), C.byref(u)))
+        return r.value, p.value, u.value
+
+    def close(self):
+        if self.handle:
+            self.lib.wind_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

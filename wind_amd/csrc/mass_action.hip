@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "kernels_dense.h"
 #include "kernels_mass_action.h"
 #include "wind_internal.h"
 
@@ -59,6 +60,13 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
         j_ent.insert(j_ent.end(), kv.second.begin(), kv.second.end());
         j_ptr.push_back((int32_t)j_ent.size());
     }
+    std::vector<std::vector<int32_t>> rows(N);
+    for (size_t z = 0; z < j_pos.size(); z++) rows[j_pos[z] % N].push_back((int32_t)z);
+    std::vector<int32_t> r_ptr(N + 1, 0), r_z;
+    for (int i = 0; i < N; i++) {
+        r_z.insert(r_z.end(), rows[i].begin(), rows[i].end());
+        r_ptr[i + 1] = (int32_t)r_z.size();
+    }
     MassActionDevice &d = n->ma;
     d.N = N;
     d.R = R;
@@ -70,30 +78,68 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     if (!rc) rc = upload((void **)&d.j_pos, j_pos.data(), sizeof(int32_t) * j_pos.size());
     if (!rc) rc = upload((void **)&d.j_ptr, j_ptr.data(), sizeof(int32_t) * j_ptr.size());
     if (!rc) rc = upload((void **)&d.j_ent, j_ent.data(), sizeof(int32_t) * j_ent.size());
+    if (!rc) rc = upload((void **)&d.r_ptr, r_ptr.data(), sizeof(int32_t) * r_ptr.size());
+    if (!rc) rc = upload((void **)&d.r_z, r_z.data(), sizeof(int32_t) * r_z.size());
     return rc;
 }
 
 void mass_action_release(wind_network *n) {
     MassActionDevice &d = n->ma;
-    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent};
+    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = MassActionDevice();
 }
 
+template <int METHOD, int PREC>
+int launch_dense(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, int ir, size_t lds, int T,
+                 const SolverChoice &sc, hipStream_t st) {
+    auto kfn = ma_dense_integrate<METHOD, PREC>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, T, lds);
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys);
+    hipLaunchKernelGGL(kfn, dim3(grid), dim3(T), lds, st, a, d, ir);
+    e = hipGetLastError();
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    return WIND_OK;
+}
+
 template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
-                       int kernel, hipStream_t st) {
+                       int kernel, const SolverChoice &sc, hipStream_t st) {
     if (kernel == WIND_KERNEL_LANE)
         return wind_fail(WIND_ERR_UNSUPPORTED, "mass-action networks run on the block kernel");
     const MassActionDevice &d = net->ma;
     const int N = d.N;
     const bool fast = mode == WIND_MODE_FAST;
-    const int T = N <= 64 ? 64 : 256;
     int dev = 0;
     hipGetDevice(&dev);
     int max_lds = 0;
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (fast && (N + 15) / 16 * 16 <= 192) {
+        const size_t lds = dense_lds_bytes(N, d.R);
+        if (lds + 128 <= (size_t)max_lds) {
+            int T = ((N + 15) & ~15) <= 64 ? 64 : 256;
+            if (sc.block == 64 || sc.block == 128 || sc.block == 256) T = sc.block;
+            const SolverChoice &g = sc;
+            int solver = sc.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : sc.solver;
+            switch (solver) {
+            case WIND_SOLVER_VALU: return launch_dense<METHOD, PREC_VALU>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
+            case WIND_SOLVER_MFMA_F32:
+                return launch_dense<METHOD, PREC_MFMA_F32>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
+            case WIND_SOLVER_MFMA_BF16:
+                return launch_dense<METHOD, PREC_MFMA_BF16>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
+            default:
+                return launch_dense<METHOD, PREC_MFMA_BF16X3>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
+            }
+        }
+    }
+    const int T = N <= 64 ? 64 : 256;
     const size_t base = sizeof(float) * (5 * (size_t)N + 3 * (size_t)d.R);
     const size_t mats = sizeof(float) * (fast ? 1 : 2) * (size_t)N * N;
     const bool in_lds = base + mats + 64 <= (size_t)max_lds;
@@ -108,8 +154,7 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, T, lds);
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    per_cu = std::max(per_cu, 1);
-    const int grid = (int)std::max(1LL, std::min((long long)num_cu * per_cu, (long long)a.nsys));
+    const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys);
     float *scratch = nullptr;
     if (!in_lds) {
         const size_t need = sizeof(float) * 2 * (size_t)N * N * grid;
@@ -130,6 +175,6 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
 }
 
 template int launch_mass_action<WIND_SIE>(int, const wind_network *, const IntegrateArgs &, int, int,
-                                          hipStream_t);
+                                          const SolverChoice &, hipStream_t);
 template int launch_mass_action<WIND_RK2>(int, const wind_network *, const IntegrateArgs &, int, int,
-                                          hipStream_t);
+                                          const SolverChoice &, hipStream_t);

@@ -12,8 +12,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "kernels_small.h"
@@ -50,7 +52,18 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Pool constants, equations, steps, status, counters;
+    // batch execution
+    std::vector<hipStream_t> bucket_streams;
+    std::vector<hipEvent_t> bucket_events;
+    hipEvent_t fork = nullptr;
+    Pool tune_state, batch_counters;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
 };
+
+__global__ void zero_u64(unsigned long long *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
 
 Context &ctx() {
     static Context c;
@@ -82,36 +95,33 @@ int reserve(Pool &p, size_t bytes) {
 }
 
 template <class K>
-int persistent_grid(const Context &c, K kernel, int block, int nsys, int *grid) {
+int persistent_grid(const Context &c, K kernel, int block, int nsys, int *grid, const SolverChoice &sc) {
     int per_cu = 0;
     WIND_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
-    if (per_cu < 1) per_cu = 1;
-    long long want = (long long)c.num_cu * per_cu;
-    long long need = ((long long)nsys + block - 1) / block;
-    *grid = (int)std::max(1LL, std::min(want, need));
+    *grid = scaled_grid(c.num_cu, per_cu, sc, nsys, block);
     return WIND_OK;
 }
 
 template <class Net, int METHOD>
 int launch_small(const Context &c, const wind_network *net, const IntegrateArgs &a, int mode,
-                 int kernel, hipStream_t st) {
+                 int kernel, const SolverChoice &sc, hipStream_t st) {
     typename Net::Params p{};
     if constexpr (sizeof(typename Net::Params) > 1) std::memcpy(&p, net->params.data(), sizeof(p));
     if (kernel == WIND_KERNEL_BLOCK) {
         if (mode != WIND_MODE_EXACT)
             return fail(WIND_ERR_UNSUPPORTED, "the block kernel implements exact mode only");
-        int grid = (int)std::min<long long>(a.nsys, (long long)c.num_cu * 32);
+        const int grid = scaled_grid(c.num_cu, 32, sc, a.nsys);
         hipLaunchKernelGGL((block_integrate<Net, METHOD>), dim3(grid), dim3(64), 0, st, a, p);
     } else if (mode == WIND_MODE_EXACT) {
         auto k = lane_integrate<Net, METHOD, MathExact>;
         int grid = 0;
-        int rc = persistent_grid(c, k, 256, a.nsys, &grid);
+        int rc = persistent_grid(c, k, 256, a.nsys, &grid, sc);
         if (rc) return rc;
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     } else {
         auto k = lane_integrate<Net, METHOD, MathFast>;
         int grid = 0;
-        int rc = persistent_grid(c, k, 256, a.nsys, &grid);
+        int rc = persistent_grid(c, k, 256, a.nsys, &grid, sc);
         if (rc) return rc;
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     }
@@ -121,12 +131,12 @@ int launch_small(const Context &c, const wind_network *net, const IntegrateArgs 
 
 template <int METHOD>
 int dispatch(const Context &c, const wind_network *net, const IntegrateArgs &a, int mode,
-             int kernel, hipStream_t st) {
+             int kernel, const SolverChoice &sc, hipStream_t st) {
     switch (net->builtin) {
-    case WIND_NET_NR_TEST: return launch_small<NetNRTest, METHOD>(c, net, a, mode, kernel, st);
-    case WIND_NET_KATZ96: return launch_small<NetKatz96, METHOD>(c, net, a, mode, kernel, st);
-    case WIND_NET_HHE6: return launch_small<NetHHe6, METHOD>(c, net, a, mode, kernel, st);
-    case WIND_NET_MASS_ACTION: return launch_mass_action<METHOD>(c.num_cu, net, a, mode, kernel, st);
+    case WIND_NET_NR_TEST: return launch_small<NetNRTest, METHOD>(c, net, a, mode, kernel, sc, st);
+    case WIND_NET_KATZ96: return launch_small<NetKatz96, METHOD>(c, net, a, mode, kernel, sc, st);
+    case WIND_NET_HHE6: return launch_small<NetHHe6, METHOD>(c, net, a, mode, kernel, sc, st);
+    case WIND_NET_MASS_ACTION: return launch_mass_action<METHOD>(c.num_cu, net, a, mode, kernel, sc, st);
     default: return fail(WIND_ERR_UNSUPPORTED, "unknown network");
     }
 }
@@ -134,6 +144,12 @@ int dispatch(const Context &c, const wind_network *net, const IntegrateArgs &a, 
 }  // namespace
 
 int wind_fail(int code, const char *msg) { return fail(code, msg); }
+
+static inline uint32_t __float_as_uint_host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
 
 extern "C" {
 
@@ -232,6 +248,9 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
         return fail(WIND_ERR_ARGUMENT, "mode must be WIND_MODE_EXACT or WIND_MODE_FAST");
     if (opts.controller != WIND_CTRL_REFERENCE && opts.controller != WIND_CTRL_ADAPTIVE)
         return fail(WIND_ERR_ARGUMENT, "controller must be WIND_CTRL_REFERENCE or WIND_CTRL_ADAPTIVE");
+    if (opts.solver < WIND_SOLVER_AUTO || opts.solver > WIND_SOLVER_MFMA_BF16X3)
+        return fail(WIND_ERR_ARGUMENT, "unknown solver");
+    if (opts.ir_sweeps < 0 || opts.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
     if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
     if (Neqn != net->N)
         return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network (N = " +
@@ -296,8 +315,11 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     a.counters = (unsigned long long *)c.counters.ptr;
 
     WIND_HIP(hipEventRecord(c.ev0, st));
-    rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, opts.mode, opts.kernel, st)
-                                 : dispatch<WIND_RK2>(c, net, a, opts.mode, opts.kernel, st);
+    SolverChoice sc;
+    sc.solver = opts.solver;
+    sc.ir_sweeps = opts.ir_sweeps;
+    rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, opts.mode, opts.kernel, sc, st)
+                                 : dispatch<WIND_RK2>(c, net, a, opts.mode, opts.kernel, sc, st);
     if (rc) return rc;
     WIND_HIP(hipEventRecord(c.ev1, st));
 
@@ -319,3 +341,298 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
 }
 
 }  // extern "C"
+
+namespace {
+
+struct TuneKey {
+    const wind_network *net;
+    int method, mode, controller, solver, ir;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(net, method, mode, controller, solver, ir) <
+               std::tie(o.net, o.method, o.mode, o.controller, o.solver, o.ir);
+    }
+};
+struct TuneVal {
+    SolverChoice sc;
+    double sec_per_sys = 0.0;
+};
+std::map<TuneKey, TuneVal> g_tuned;
+
+int launch_bucket(const Context &c, const wind_network *net, const IntegrateArgs &a, const wind_options &o,
+                  const SolverChoice &sc, hipStream_t st) {
+    return o.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, o.mode, WIND_KERNEL_AUTO, sc, st)
+                                : dispatch<WIND_RK2>(c, net, a, o.mode, WIND_KERNEL_AUTO, sc, st);
+}
+
+IntegrateArgs bucket_args(const wind_bucket &b, float tnow, float tend, int nfix, float ABS, float REL,
+                          const wind_options &o, unsigned long long *counters) {
+    IntegrateArgs a;
+    a.tnow = tnow;
+    a.tend = tend;
+    a.h0 = (tend - tnow) / (float)nfix;
+    a.ABS = ABS;
+    a.REL = REL;
+    a.controller = o.controller;
+    a.constants = b.constants;
+    a.equations = b.equations;
+    a.nsys = b.nsys;
+    a.const_stride = b.const_stride > 0 ? b.const_stride : b.net->nconst;
+    a.max_attempts = o.max_attempts > 0 ? o.max_attempts : kDefaultMaxAttempts;
+    a.steps_out = b.steps_out;
+    a.status_out = b.status_out;
+    a.index = nullptr;
+    a.counters = counters;
+    return a;
+}
+
+// Candidate kernel shapes for a bucket; the first is the default.
+std::vector<SolverChoice> candidates(const wind_network *net, const wind_options &o) {
+    std::vector<SolverChoice> v;
+    SolverChoice base;
+    base.solver = o.solver;
+    base.ir_sweeps = o.ir_sweeps;
+    v.push_back(base);
+    if (net->builtin != WIND_NET_MASS_ACTION || o.mode != WIND_MODE_FAST) return v;
+    const int NP = (net->N + 15) & ~15;
+    if (NP > 192) return v;
+    v.clear();
+    const int solvers[2] = {WIND_SOLVER_MFMA_F32, WIND_SOLVER_VALU};
+    const int blocks[3] = {64, 128, 256};
+    for (int sv : solvers) {
+        if (o.solver != WIND_SOLVER_AUTO && sv != o.solver) continue;
+        for (int blk : blocks) {
+            if (NP > 64 && blk == 64) continue;
+            SolverChoice sc;
+            sc.solver = sv;
+            sc.ir_sweeps = o.ir_sweeps;
+            sc.block = blk;
+            v.push_back(sc);
+        }
+    }
+    if (v.empty()) v.push_back(base);
+    return v;
+}
+
+int tune_bucket(Context &c, const wind_bucket &b, float tnow, float tend, int nfix, float ABS, float REL,
+                const wind_options &o, hipStream_t st, TuneVal *out) {
+    TuneKey key{b.net, o.method, o.mode, o.controller, o.solver, o.ir_sweeps};
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+        *out = it->second;
+        return WIND_OK;
+    }
+    const int N = b.net->N;
+    const int ns = std::min(b.nsys, N <= 16 ? 16384 : (N <= 64 ? 4096 : 1024));
+    const size_t bytes = (size_t)ns * N * sizeof(float);
+    int rc = reserve(c.tune_state, bytes);
+    if (!rc) rc = reserve(c.counters, 4 * sizeof(unsigned long long));
+    if (rc) return rc;
+    wind_bucket sb = b;
+    sb.nsys = ns;
+    sb.equations = (float *)c.tune_state.ptr;
+    sb.steps_out = nullptr;
+    sb.status_out = nullptr;
+    std::vector<SolverChoice> cand = candidates(b.net, o);
+    TuneVal best;
+    best.sec_per_sys = 1e300;
+    for (const SolverChoice &sc : cand) {
+        WIND_HIP(hipMemcpyAsync(c.tune_state.ptr, b.equations, bytes, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)c.counters.ptr, 4);
+        IntegrateArgs a = bucket_args(sb, tnow, tend, nfix, ABS, REL, o, (unsigned long long *)c.counters.ptr);
+        WIND_HIP(hipEventRecord(c.ev0, st));
+        rc = launch_bucket(c, b.net, a, o, sc, st);
+        if (rc) return rc;
+        WIND_HIP(hipEventRecord(c.ev1, st));
+        WIND_HIP(hipEventSynchronize(c.ev1));
+        float ms = 0.0f;
+        WIND_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        const double sps = ms * 1e-3 / ns;
+        if (sps < best.sec_per_sys) {
+            best.sec_per_sys = sps;
+            best.sc = sc;
+        }
+    }
+    g_tuned[key] = best;
+    *out = best;
+    return WIND_OK;
+}
+
+}  // namespace
+
+extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tnow, float tend, int nfix,
+                                    float ABSOLUTE, float RELATIVE, const wind_options *opts_in) {
+    g_last_error.clear();
+    wind_options o;
+    if (opts_in)
+        o = *opts_in;
+    else
+        wind_options_init(&o);
+    if (o.total_steps) *o.total_steps = 0;
+    if (o.kernel_ms) *o.kernel_ms = 0.0f;
+    if (nb < 0 || (nb > 0 && !buckets)) return fail(WIND_ERR_ARGUMENT, "bad bucket list");
+    if (o.method != WIND_SIE && o.method != WIND_RK2) return fail(WIND_ERR_ARGUMENT, "bad method");
+    if (o.mode != WIND_MODE_EXACT && o.mode != WIND_MODE_FAST) return fail(WIND_ERR_ARGUMENT, "bad mode");
+    if (o.controller != WIND_CTRL_REFERENCE && o.controller != WIND_CTRL_ADAPTIVE)
+        return fail(WIND_ERR_ARGUMENT, "bad controller");
+    if (o.solver < WIND_SOLVER_AUTO || o.solver > WIND_SOLVER_MFMA_BF16X3) return fail(WIND_ERR_ARGUMENT, "bad solver");
+    if (o.ir_sweeps < 0 || o.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
+    if (nfix < 0) return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0");
+    for (int i = 0; i < nb; i++) {
+        const wind_bucket &b = buckets[i];
+        if (!b.net || b.nsys < 0 || (b.nsys > 0 && (!b.constants || !b.equations)))
+            return fail(WIND_ERR_ARGUMENT, "bucket " + std::to_string(i) + ": bad network, size or pointers");
+    }
+    if (nb == 0 || !(tnow < tend)) return WIND_OK;
+    Context &c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    int rc = ensure_ready(c);
+    if (rc) return rc;
+    hipStream_t st = o.stream ? (hipStream_t)o.stream : c.stream;
+
+    // Tune each bucket (cached), then split the device by estimated work.
+    std::vector<TuneVal> tv(nb);
+    double work = 0.0;
+    for (int i = 0; i < nb; i++) {
+        if (buckets[i].nsys == 0) continue;
+        if (o.flags & WIND_FLAG_NO_AUTOTUNE) {
+            tv[i].sc = candidates(buckets[i].net, o)[0];
+            tv[i].sec_per_sys = 1.0;
+        } else {
+            rc = tune_bucket(c, buckets[i], tnow, tend, nfix, ABSOLUTE, RELATIVE, o, st, &tv[i]);
+            if (rc) return rc;
+        }
+        work += tv[i].sec_per_sys * buckets[i].nsys;
+    }
+    std::vector<SolverChoice> sc(nb);
+    for (int i = 0; i < nb; i++) {
+        sc[i] = tv[i].sc;
+        if (buckets[i].nsys == 0 || work <= 0.0) continue;
+        // each bucket's persistent grid gets the fraction of the device its work needs
+        sc[i].share = (float)std::min(1.0, std::max(1.0 / c.num_cu, tv[i].sec_per_sys * buckets[i].nsys / work));
+    }
+
+    while ((int)c.bucket_streams.size() < nb) {
+        hipStream_t s2;
+        hipEvent_t e2;
+        WIND_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        WIND_HIP(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+        c.bucket_streams.push_back(s2);
+        c.bucket_events.push_back(e2);
+    }
+    if (!c.fork) WIND_HIP(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
+    rc = reserve(c.batch_counters, (size_t)nb * 4 * sizeof(unsigned long long));
+    if (rc) return rc;
+    unsigned long long *cnt = (unsigned long long *)c.batch_counters.ptr;
+
+    auto enqueue = [&]() -> int {
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, cnt, nb * 4);
+        WIND_HIP(hipEventRecord(c.fork, st));
+        for (int i = 0; i < nb; i++) {
+            if (buckets[i].nsys == 0) continue;
+            hipStream_t bs = c.bucket_streams[i];
+            WIND_HIP(hipStreamWaitEvent(bs, c.fork, 0));
+            IntegrateArgs a = bucket_args(buckets[i], tnow, tend, nfix, ABSOLUTE, RELATIVE, o, cnt + 4 * i);
+            int r = launch_bucket(c, buckets[i].net, a, o, sc[i], bs);
+            if (r) return r;
+            WIND_HIP(hipEventRecord(c.bucket_events[i], bs));
+            WIND_HIP(hipStreamWaitEvent(st, c.bucket_events[i], 0));
+        }
+        return WIND_OK;
+    };
+
+    WIND_HIP(hipEventRecord(c.ev0, st));
+    if (o.flags & WIND_FLAG_GRAPH) {
+        std::vector<uint64_t> key;
+        auto push = [&](uint64_t v) { key.push_back(v); };
+        push((uint64_t)(uintptr_t)st);
+        push(((uint64_t)__float_as_uint_host(tnow) << 32) | __float_as_uint_host(tend));
+        push(((uint64_t)__float_as_uint_host(ABSOLUTE) << 32) | __float_as_uint_host(RELATIVE));
+        push(((uint64_t)(uint32_t)nfix << 32) | (uint32_t)o.method | ((uint64_t)o.mode << 8) |
+             ((uint64_t)o.controller << 16));
+        push((uint64_t)o.max_attempts);
+        for (int i = 0; i < nb; i++) {
+            const wind_bucket &b = buckets[i];
+            push((uint64_t)(uintptr_t)b.net);
+            push((uint64_t)(uintptr_t)b.constants);
+            push((uint64_t)(uintptr_t)b.equations);
+            push((uint64_t)(uintptr_t)b.steps_out);
+            push((uint64_t)(uintptr_t)b.status_out);
+            push(((uint64_t)(uint32_t)b.nsys << 32) | (uint32_t)b.const_stride);
+            push(((uint64_t)(uint32_t)sc[i].solver << 48) | ((uint64_t)(uint32_t)sc[i].block << 32) |
+                 ((uint64_t)(uint32_t)sc[i].ir_sweeps << 24) | (uint32_t)sc[i].grid);
+            push((uint64_t)__float_as_uint_host(sc[i].share));
+        }
+        push((uint64_t)(uintptr_t)cnt);
+        if (!c.graph_exec || key != c.graph_key) {
+            if (c.graph_exec) {
+                (void)hipGraphExecDestroy(c.graph_exec);
+                c.graph_exec = nullptr;
+            }
+            hipGraph_t g;
+            WIND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+            rc = enqueue();
+            hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc) return rc;
+            if (ec != hipSuccess) return fail(WIND_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+            WIND_HIP(hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            c.graph_key = key;
+        }
+        WIND_HIP(hipGraphLaunch(c.graph_exec, st));
+    } else {
+        rc = enqueue();
+        if (rc) return rc;
+    }
+    WIND_HIP(hipEventRecord(c.ev1, st));
+    std::vector<unsigned long long> host(nb * 4);
+    WIND_HIP(hipMemcpyAsync(host.data(), cnt, host.size() * 8, hipMemcpyDeviceToHost, st));
+    WIND_HIP(hipStreamSynchronize(st));
+    if (o.kernel_ms) WIND_HIP(hipEventElapsedTime(o.kernel_ms, c.ev0, c.ev1));
+    long long total = 0;
+    for (int i = 0; i < nb; i++) total += (long long)host[4 * i + 1];
+    if (o.total_steps) *o.total_steps = total;
+    return WIND_OK;
+}
+
+int wind_launch_async(const wind_network *net, const IntegrateArgs &a, int method, int mode, int kernel,
+                      const SolverChoice &sc, hipStream_t st) {
+    Context &c = ctx();
+    {
+        std::lock_guard<std::mutex> lock(c.mu);
+        int rc = ensure_ready(c);
+        if (rc) return rc;
+    }
+    if (net->builtin == WIND_NET_MASS_ACTION && mode == WIND_MODE_EXACT) {
+        const int N = net->N;
+        int dev = 0, max_lds = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        const size_t need = sizeof(float) * (5 * (size_t)N + 3 * (size_t)net->ma.R + 2 * (size_t)N * N) + 64;
+        if (need > (size_t)max_lds)
+            return fail(WIND_ERR_UNSUPPORTED, "exact mode for this network needs a shared scratch buffer");
+    }
+    if (net->builtin == WIND_NET_MASS_ACTION && mode == WIND_MODE_FAST && ((net->N + 15) & ~15) > 192) {
+        const int N = net->N;
+        int dev = 0, max_lds = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        const size_t need = sizeof(float) * (5 * (size_t)N + 3 * (size_t)net->ma.R + (size_t)N * N) + 64;
+        if (need > (size_t)max_lds)
+            return fail(WIND_ERR_UNSUPPORTED, "this network needs a shared scratch buffer");
+    }
+    return method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, mode, kernel, sc, st)
+                              : dispatch<WIND_RK2>(c, net, a, mode, kernel, sc, st);
+}
+
+int wind_device_cus(int *num_cu) {
+    Context &c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    int rc = ensure_ready(c);
+    if (rc) return rc;
+    *num_cu = c.num_cu;
+    return WIND_OK;
+}
+
+void wind_zero_counters(unsigned long long *p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, p, n);
+}

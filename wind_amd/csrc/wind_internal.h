@@ -21,6 +21,8 @@ struct MassActionDevice {
     int32_t *j_pos = nullptr;  // [nnz]  element index e = col*N + row
     int32_t *j_ptr = nullptr;  // [nnz+1]
     int32_t *j_ent = nullptr;  //        ((r*2 + use_db)*2) + (sign < 0)
+    int32_t *r_ptr = nullptr;  // [N+1]  non-zeros by row (for residuals)
+    int32_t *r_z = nullptr;    //        indices into j_pos
 };
 
 }  // namespace wind
@@ -39,6 +41,35 @@ struct wind_network {
 int wind_fail(int code, const char *msg);
 int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const float *k);
 void mass_action_release(wind_network *n);
+struct SolverChoice {
+    int solver = 0;     // wind_solver
+    int ir_sweeps = 0;  // refinement sweeps (dense path)
+    int block = 0;      // threads per workgroup (dense path); 0 = by N
+    int grid = 0;       // workgroups; 0 = fill the device
+    float share = 0.0f; // fraction of a full-device grid to use (batches); 0 = all
+};
+
+// Workgroups for a persistent kernel: a full device (per_cu per CU), scaled
+// by a batch share, capped by an explicit grid and by the system count.
+inline int scaled_grid(int num_cu, int per_cu, const SolverChoice &sc, int nsys, int per_block_systems = 1) {
+    long long full = (long long)num_cu * (per_cu < 1 ? 1 : per_cu);
+    if (sc.share > 0.0f && sc.share < 1.0f) full = (long long)(full * (double)sc.share + 0.999);
+    if (sc.grid > 0 && sc.grid < full) full = sc.grid;
+    long long need = ((long long)nsys + per_block_systems - 1) / per_block_systems;
+    if (full > need) full = need;
+    return (int)(full < 1 ? 1 : full);
+}
 template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const wind::IntegrateArgs &a, int mode,
-                       int kernel, hipStream_t st);
+                       int kernel, const SolverChoice &sc, hipStream_t st);
+
+// Enqueues one integration kernel on `st` without synchronising (device
+// pointers in `a`; a.counters must point at 4 zeroed 64-bit counters).
+// Fails with WIND_ERR_UNSUPPORTED for paths that need the shared scratch
+// buffer (exact mode on mass-action networks too large for LDS).
+int wind_launch_async(const wind_network *net, const wind::IntegrateArgs &a, int method, int mode, int kernel,
+                      const SolverChoice &sc, hipStream_t st);
+// Device CU count (initialises the library context).
+int wind_device_cus(int *num_cu);
+// Fills n 64-bit words with zero on `st` (graph- and capture-safe).
+void wind_zero_counters(unsigned long long *p, int n, hipStream_t st);

@@ -108,3 +108,22 @@ def mixed_population(nsys: int, seed: int):
     """Config 4: N drawn from MIXED_SIZES with MIXED_WEIGHTS; returns the N of each system."""
     rng = np.random.default_rng(seed)
     return rng.choice(np.array(MIXED_SIZES), size=nsys, p=np.array(MIXED_WEIGHTS))
+
+
+MIXED_SCALE_RANGE = (0.1, 10.0)
+
+
+def mixed_workload(nsys: int, seed: int = 4):
+    """Config 4: buckets of random mass-action networks, one network per N in
+    MIXED_SIZES (seeded by N), systems assigned to N with MIXED_WEIGHTS, per-
+    system rate scales logU(MIXED_SCALE_RANGE) so step counts vary widely.
+    Returns a list of dicts {N, rx, k, y0, scale, index} (index = positions
+    of the bucket's systems in the mixed population)."""
+    Ns = mixed_population(nsys, seed)
+    out = []
+    for N in MIXED_SIZES:
+        idx = np.nonzero(Ns == N)[0]
+        rx, k = mass_action_network(N, seed=1000 + N)
+        y0, sc = mass_action_systems(len(idx), N, seed=2000 + N + seed, scale_range=MIXED_SCALE_RANGE)
+        out.append({"N": N, "rx": rx, "k": k, "y0": y0, "scale": sc, "index": idx})
+    return out
