@@ -70,7 +70,7 @@ KERNELS = [api.KERNEL_LANE, api.KERNEL_BLOCK]
 @pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("method", [api.SIE, api.RK2])
 def test_nr_test_exact(kernel, method):
-    """Config 1 workload shape: NR_test, t in [0, 1], dt0 = 1e-3, tol 1e-3."""
+    """NR_test workload shape:, t in [0, 1], dt0 = 1e-3, tol 1e-3."""
     y0, c, _ = W.nr_test_systems(1024, seed=3)
     net = api.Network.builtin(api.NET_NR_TEST)
     g = run_gpu(net, 0.0, 1.0, 1000, c, y0, 1024, 1e-3, method, kernel)
@@ -102,7 +102,7 @@ def test_katz96_exact(kernel, method, tol):
 @pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("method", [api.SIE, api.RK2])
 def test_hhe6_exact(kernel, method):
-    """hhe6 (config 2 network) over 1 kyr; includes systems whose first
+    """hhe6 network over 1 kyr; includes systems whose first
     1-kyr attempt overflows to NaN, which both sides must reproduce."""
     n = 256
     p = W.hhe6_params()

@@ -69,7 +69,7 @@ def test_take_step_reduces_to_explicit_euler_and_diagonal_form():
 
 
 def test_nr_test_against_closed_form():
-    """Config 1: NR_test over t in [0, 1] against y1 = 2e^{-at} - e^{-1000at},
+    """NR_test over t in [0, 1] against y1 = 2e^{-at} - e^{-1000at},
     y2 = -e^{-at} + e^{-1000at}.  The reference controller at tol 1e-3 is
     first-order accurate (about 1.3% at t = 1); the adaptive controller at
     tol 1e-7 is within 1e-3 (float32 time and state)."""

@@ -4,22 +4,22 @@
 // 16), factored by a blocked right-looking LU without pivoting (the
 // reference's solve is unpivoted too, gje_gold.c:56-134).
 //
-// Per 16-column panel: the panel is factored in place (16 pivot steps), the
-// 16-row block of U to its right is solved against the unit lower diagonal
-// block, and the trailing matrix is updated A22 -= L21 * U12 in 16x16 tiles
-// spread over the workgroup's waves.  The tile product runs on the matrix
-// cores:
+// Per 16-column panel one wave factors the 16x16 diagonal block and inverts
+// its two triangular factors; the panel blocks L21 = A21 Uinv and
+// U12 = Linv A12 and the trailing update A22 -= L21 U12 are 16x16 tile
+// products spread over the workgroup's waves.  The tile products run on the
+// matrix cores:
 //   PREC_VALU      fused multiply-adds on the vector ALU (no MFMA), reference point
 //   PREC_MFMA_F32  v_mfma_f32_16x16x4f32, four per tile (K = 16)
 //   PREC_MFMA_BF16 v_mfma_f32_16x16x16bf16_1k, one per tile: BF16 inputs, FP32 accumulation
 //   PREC_MFMA_BF16X3 three BF16 products (hi*hi + hi*lo + lo*hi) of the split operands
-// The 16x16 diagonal block of each panel is factored in one wave with
-// register broadcasts; the rows below it are solved against U11 in parallel.
-// Forward and back substitution go by 16-row blocks.  Optional iterative
-// refinement recomputes the residual r = f - (I - hJ)x in double (J x
-// gathered row by row) and solves for a correction with the same factors.
-// Every step is deterministic: a system's result does not depend on what
-// else runs on the device.
+// (the panel products always use FP32: BF16 applies to the trailing update).
+// The triangular solves run in one wave with the right-hand side in
+// registers, one 16-row block at a time through the stored diagonal-block
+// inverses.  Optional iterative refinement recomputes the residual
+// r = f - (I - hJ)x in double (J x gathered row by row) and solves for a
+// correction with the same factors.  Every step is deterministic: a
+// system's result does not depend on what else runs on the device.
 #pragma once
 #include "wind_device.h"
 #include "wind_internal.h"
@@ -38,36 +38,39 @@ __device__ inline short to_bf16_rne(float x) {
 }
 __device__ inline float from_bf16(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
-// C tile (16x16) of the trailing update: rows r0.., columns c0.., K = the
-// 16 panel columns k0..k0+15.  C -= L21 * U12.
-template <int PREC>
-__device__ inline void tile_update(float *A, int S, int r0, int c0, int k0, int lane) {
-    const int lr = lane & 15, lq = lane >> 4;
-    f32x4 c;
-#pragma unroll
-    for (int r = 0; r < 4; r++) c[r] = A[(r0 + 4 * lq + r) * S + c0 + lr];
+__device__ inline float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// c += X * Y for 16x16 tiles, X(m, k) and Y(k, n) element functors.  Output
+// layout (the MFMA accumulator layout): lane l holds rows 4(l/16) + r,
+// r = 0..3, of column l % 16.
+template <int PREC, class FX, class FY>
+__device__ inline f32x4 tile_mma(FX X, FY Y, f32x4 c, int lane) {
+    const int lr = lane & 15, g = lane >> 4;
     if (PREC == PREC_VALU) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = r0 + 4 * lq + r;
             float acc = c[r];
 #pragma unroll
-            for (int k = 0; k < 16; k++) acc = fmaf(-A[row * S + k0 + k], A[(k0 + k) * S + c0 + lr], acc);
+            for (int k = 0; k < 16; k++) acc = fmaf(X(4 * g + r, k), Y(k, lr), acc);
             c[r] = acc;
         }
     } else if (PREC == PREC_MFMA_F32) {
+        float xa[4], yb[4];
 #pragma unroll
-        for (int kk = 0; kk < 16; kk += 4) {
-            const float av = -A[(r0 + lr) * S + k0 + kk + lq];
-            const float bv = A[(k0 + kk + lq) * S + c0 + lr];
-            c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, c, 0, 0, 0);
+        for (int q = 0; q < 4; q++) {
+            xa[q] = X(lr, 4 * q + g);
+            yb[q] = Y(4 * q + g, lr);
         }
+#pragma unroll
+        for (int q = 0; q < 4; q++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[q], yb[q], c, 0, 0, 0);
     } else {
         bf16x4_t ah, bh, al, bl;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float av = -A[(r0 + lr) * S + k0 + 4 * lq + q];
-            const float bv = A[(k0 + 4 * lq + q) * S + c0 + lr];
+            const float av = X(lr, 4 * g + q);
+            const float bv = Y(4 * g + q, lr);
             ah[q] = to_bf16_rne(av);
             bh[q] = to_bf16_rne(bv);
             if (PREC == PREC_MFMA_BF16X3) {
@@ -81,130 +84,207 @@ __device__ inline void tile_update(float *A, int S, int r0, int c0, int k0, int 
         }
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
     }
-#pragma unroll
-    for (int r = 0; r < 4; r++) A[(r0 + 4 * lq + r) * S + c0 + lr] = c[r];
+    return c;
 }
 
-// In-place LU (unit lower L below the diagonal, U on and above) of the
-// NP x NP matrix at A with row stride S.  Called by the whole workgroup.
+__device__ inline float lane_read(float v, int src_lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+}
+
+// Replaces the 16x16 diagonal block at (k0, k0) by its inverse (in-place
+// Gauss-Jordan without pivoting).  Executed by one whole wave: lane l holds
+// row l % 16, columns 4(l / 16) .. 4(l / 16) + 3; the pivot row and the
+// pivot column are fetched across lanes with ds_bpermute.
+__device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
+    const int r = lane & 15, g = lane >> 4;
+    float a[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) a[c] = A[(k0 + r) * S + k0 + 4 * g + c];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int kg = k >> 2, kc = k & 3;
+        float pv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) pv[c] = lane_read(a[c], k + 16 * g);
+        const float f = lane_read(a[kc], r + 16 * kg);
+        const float rp = __builtin_amdgcn_rcpf(bcast(a[kc], k + 16 * kg));
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const float prow = pv[c] * rp;
+            const bool pc = 4 * g + c == k;
+            if (r == k)
+                a[c] = pc ? rp : prow;
+            else
+                a[c] = pc ? -f * rp : fmaf(-f, prow, a[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) A[(k0 + r) * S + k0 + 4 * g + c] = a[c];
+}
+
+// Cost of one diagonal-block inversion in trailing-tile updates (balances
+// wave 0's share of the trailing update).
+constexpr int kDiagTiles = 4;
+
+// Trailing update C -= A21 W for tiles [lo, hi) (row-major over nt x nt
+// tiles of the trailing matrix at (kend, kend)); with FP32 MFMA two tiles go
+// at a time so that their load latencies and MFMA chains overlap.
+template <int PREC>
+__device__ inline void trailing_tiles(float *A, int S, int k0, int kend, int nt, int lo, int hi, int lane) {
+    const int lr = lane & 15, g = lane >> 4;
+    int tile = lo;
+    if (PREC == PREC_MFMA_F32) {
+        for (; tile + 1 < hi; tile += 2) {
+            const int rb0 = tile / nt, rb1 = (tile + 1) / nt;
+            const int r0 = kend + (rb0 << 4), c0 = kend + ((tile - rb0 * nt) << 4);
+            const int r1 = kend + (rb1 << 4), c1 = kend + ((tile + 1 - rb1 * nt) << 4);
+            f32x4 d0, d1;
+            float a0[4], b0[4], a1[4], b1[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                d0[q] = A[(r0 + 4 * g + q) * S + c0 + lr];
+                d1[q] = A[(r1 + 4 * g + q) * S + c1 + lr];
+                a0[q] = -A[(r0 + lr) * S + k0 + 4 * q + g];
+                b0[q] = A[(k0 + 4 * q + g) * S + c0 + lr];
+                a1[q] = -A[(r1 + lr) * S + k0 + 4 * q + g];
+                b1[q] = A[(k0 + 4 * q + g) * S + c1 + lr];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b0[q], d0, 0, 0, 0);
+                d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b1[q], d1, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                A[(r0 + 4 * g + q) * S + c0 + lr] = d0[q];
+                A[(r1 + 4 * g + q) * S + c1 + lr] = d1[q];
+            }
+        }
+    }
+    for (; tile < hi; tile++) {
+        const int rb = tile / nt;
+        const int r0 = kend + (rb << 4), c0 = kend + ((tile - rb * nt) << 4);
+        f32x4 c;
+#pragma unroll
+        for (int q = 0; q < 4; q++) c[q] = A[(r0 + 4 * g + q) * S + c0 + lr];
+        c = tile_mma<PREC>([&](int m, int k) { return -A[(r0 + m) * S + k0 + k]; },
+                           [&](int k, int n) { return A[(k0 + k) * S + c0 + n]; }, c, lane);
+#pragma unroll
+        for (int q = 0; q < 4; q++) A[(r0 + 4 * g + q) * S + c0 + lr] = c[q];
+    }
+}
+
+// In-place blocked LU without pivoting of the NP x NP matrix at A (row
+// stride S), called by the whole workgroup, in the block form
+//   A = [I 0; A21 G I] [A11 A12; 0 A22 - A21 G A12],  G = A11^-1.
+// On return each 16x16 diagonal block holds its G, the blocks above the
+// diagonal hold W = G A12 and the blocks below hold A21 (each as updated
+// by the earlier panels).  Per panel: W = G A12 in 16x16 tile products,
+// then the trailing update A22 -= A21 W; wave 0 updates the next diagonal
+// block first and inverts it while the other waves finish the update.
 template <int PREC>
 __device__ inline void dense_lu(float *A, int NP, int S) {
     const int t = threadIdx.x, T = blockDim.x;
     const int wave = t >> 6, lane = t & 63, nw = T >> 6;
-    for (int k0 = 0; k0 < NP; k0 += 16) {
+    const int lr = lane & 15, g = lane >> 4;
+    constexpr int PP = PREC == PREC_VALU ? PREC_VALU : PREC_MFMA_F32;  // W products
+    if (wave == 0) diag_block_invert(A, S, 0, lane);
+    __syncthreads();
+    for (int k0 = 0; k0 + 16 < NP; k0 += 16) {
         const int kend = k0 + 16;
-        // Diagonal block: LU of the 16x16 block in lanes 0-15 of wave 0, lane
-        // r holding row k0 + r; pivot rows are broadcast with readlane.
-        if (t < 16) {
-            float a[16];
-#pragma unroll
-            for (int j = 0; j < 16; j++) a[j] = A[(k0 + t) * S + k0 + j];
-#pragma unroll
-            for (int kk = 0; kk < 15; kk++) {
-                float pr[16];
-#pragma unroll
-                for (int j = kk; j < 16; j++) pr[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[j]), kk));
-                if (t > kk) {
-                    const float l = a[kk] / pr[kk];
-                    a[kk] = l;
-#pragma unroll
-                    for (int j = kk + 1; j < 16; j++) a[j] = fmaf(-l, pr[j], a[j]);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 16; j++) A[(k0 + t) * S + k0 + j] = a[j];
-        }
-        __syncthreads();
-        // Rows below the diagonal block: solve l U11 = a for each row (L21 = A21 U11^-1).
-        for (int i = kend + t; i < NP; i += T) {
-            float a[16];
-#pragma unroll
-            for (int j = 0; j < 16; j++) a[j] = A[i * S + k0 + j];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) {
-                const float l = a[kk] / A[(k0 + kk) * S + k0 + kk];
-                a[kk] = l;
-#pragma unroll
-                for (int j = kk + 1; j < 16; j++) a[j] = fmaf(-l, A[(k0 + kk) * S + k0 + j], a[j]);
-            }
-#pragma unroll
-            for (int j = 0; j < 16; j++) A[i * S + k0 + j] = a[j];
-        }
-        __syncthreads();
-        if (kend >= NP) break;
-        // U12 = L11^{-1} A12, one column per thread.
-        for (int j = kend + t; j < NP; j += T) {
-            float u[16];
-#pragma unroll
-            for (int k = 0; k < 16; k++) u[k] = A[(k0 + k) * S + j];
-#pragma unroll
-            for (int k = 0; k < 16; k++)
-#pragma unroll
-                for (int r = k + 1; r < 16; r++) u[r] = fmaf(-A[(k0 + r) * S + k0 + k], u[k], u[r]);
-#pragma unroll
-            for (int k = 1; k < 16; k++) A[(k0 + k) * S + j] = u[k];
-        }
-        __syncthreads();
         const int nt = (NP - kend) >> 4;
-        for (int tile = wave; tile < nt * nt; tile += nw)
-            tile_update<PREC>(A, S, kend + ((tile / nt) << 4), kend + ((tile % nt) << 4), k0, lane);
+        for (int u = wave; u < nt; u += nw) {
+            const int c0 = kend + (u << 4);
+            const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+            const f32x4 c = tile_mma<PP>([&](int m, int k) { return A[(k0 + m) * S + k0 + k]; },
+                                         [&](int k, int n) { return A[(k0 + k) * S + c0 + n]; }, z, lane);
+            // the wave reads the whole tile into registers before writing it back
+#pragma unroll
+            for (int q = 0; q < 4; q++) A[(k0 + 4 * g + q) * S + c0 + lr] = c[q];
+        }
+        __syncthreads();
+        // Tiles 1.. are split over the waves; wave 0 first updates tile 0 (the
+        // next diagonal block) and inverts it, so it takes fewer of the rest.
+        const int ntt = nt * nt, rest = ntt - 1;
+        const int w0 = nw == 1 ? rest : max(0, (rest + kDiagTiles) / nw - kDiagTiles);
+        const int lo = wave == 0 ? 1 : 1 + w0 + (wave - 1) * (rest - w0) / (nw - 1);
+        const int hi = wave == 0 ? 1 + w0 : 1 + w0 + wave * (rest - w0) / (nw - 1);
+        if (wave == 0) {
+            trailing_tiles<PREC>(A, S, k0, kend, nt, 0, 1, lane);
+            diag_block_invert(A, S, kend, lane);
+        }
+        trailing_tiles<PREC>(A, S, k0, kend, nt, lo, hi, lane);
         __syncthreads();
     }
 }
 
-// Solves L U x = b by 16-row blocks: the 16x16 diagonal block in lanes 0-15
-// of wave 0 (register shuffles at compile-time lanes), then a parallel
-// update of the remaining rows.  b and x may alias; both in LDS.
+// Solves A x = b with the block factors left by dense_lu, in one wave
+// (lane l keeps rows l, l + 64, l + 128 in registers; NP <= 192).
+// Forward sweep per 16-row block k: c_k = G_k y_k (block values gathered by
+// readlane), written back to their lanes and broadcast, then
+// y_i -= A21_ik c_k for the rows below.  Backward sweep: x_k = c_k once the
+// blocks after k are done, then c_i -= W_ik x_k for the rows above.
+// b and x may alias; both in LDS.
 __device__ inline void dense_lu_solve(const float *A, int NP, int S, const float *b, float *x) {
-    const int t = threadIdx.x, T = blockDim.x;
-    if (x != b)
-        for (int i = t; i < NP; i += T) x[i] = b[i];
+    const int t = threadIdx.x;
+    if (t < 64) {
+        const int m = t & 15, grp = t >> 4;
+        const int NB = NP >> 4;
+        float xr[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int row = 64 * c + t;
+            xr[c] = row < NP ? b[row] : 0.0f;
+        }
+#pragma unroll
+        for (int kb = 0; kb < 12; kb++) {
+            if (kb >= NB) break;
+            const int c = kb >> 2, kq = kb & 3, k0 = kb << 4;
+            float y = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 16; j++) y = fmaf(A[(k0 + m) * S + k0 + j], bcast(xr[c], 16 * kq + j), y);
+            if (grp == kq) xr[c] = y;
+            if (kb + 1 < NB) {
+                float yk[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) yk[j] = bcast(y, j);
+#pragma unroll
+                for (int c2 = 0; c2 < 3; c2++) {
+                    const int row = 64 * c2 + t;
+                    if (c2 >= c && row >= k0 + 16 && row < NP) {
+                        float acc = xr[c2];
+#pragma unroll
+                        for (int j = 0; j < 16; j++) acc = fmaf(-A[row * S + k0 + j], yk[j], acc);
+                        xr[c2] = acc;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int kb = 11; kb > 0; kb--) {
+            if (kb >= NB) continue;
+            const int c = kb >> 2, kq = kb & 3, k0 = kb << 4;
+            float xk[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) xk[j] = bcast(xr[c], 16 * kq + j);
+#pragma unroll
+            for (int c2 = 0; c2 < 3; c2++) {
+                const int row = 64 * c2 + t;
+                if (c2 <= c && row < k0) {
+                    float acc = xr[c2];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) acc = fmaf(-A[row * S + k0 + j], xk[j], acc);
+                    xr[c2] = acc;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int row = 64 * c + t;
+            if (row < NP) x[row] = xr[c];
+        }
+    }
     __syncthreads();
-    for (int k0 = 0; k0 < NP; k0 += 16) {
-        if (t < 16) {
-            float xi = x[k0 + t];
-            float l[16];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) l[kk] = A[(k0 + t) * S + k0 + kk];
-#pragma unroll
-            for (int kk = 0; kk < 15; kk++) {
-                const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xi), kk));
-                if (t > kk) xi = fmaf(-l[kk], xk, xi);
-            }
-            x[k0 + t] = xi;
-        }
-        __syncthreads();
-        for (int i = k0 + 16 + t; i < NP; i += T) {
-            float acc = x[i];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) acc = fmaf(-A[i * S + k0 + kk], x[k0 + kk], acc);
-            x[i] = acc;
-        }
-        __syncthreads();
-    }
-    for (int k0 = NP - 16; k0 >= 0; k0 -= 16) {
-        if (t < 16) {
-            float xi = x[k0 + t];
-            float u[16];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) u[kk] = A[(k0 + t) * S + k0 + kk];
-#pragma unroll
-            for (int kk = 15; kk >= 0; kk--) {
-                if (t == kk) xi = xi / u[kk];
-                const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xi), kk));
-                if (t < kk) xi = fmaf(-u[kk], xk, xi);
-            }
-            x[k0 + t] = xi;
-        }
-        __syncthreads();
-        for (int i = t; i < k0; i += T) {
-            float acc = x[i];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) acc = fmaf(-A[i * S + k0 + kk], x[k0 + kk], acc);
-            x[i] = acc;
-        }
-        __syncthreads();
-    }
 }
 
 struct DenseLayout {
@@ -214,6 +294,8 @@ struct DenseLayout {
 };
 
 __device__ inline int dense_np(int N) { return (N + 15) & ~15; }
+// Float offset of the matrix in the LDS layout (16-byte aligned).
+__host__ __device__ inline int dense_a_offset(int NP, int R) { return (6 * NP + 5 * R + 3) & ~3; }
 
 // f = rhs(yv) and A = I - hJ (dense, padded).  Phase 1: one thread per
 // reaction stores w = k s y_a y_b, h*da = h k s y_b, h*db = h k s y_a and the
@@ -234,11 +316,15 @@ __device__ inline void dense_eval(const MassActionDevice &net, const float *yv, 
         L.ra[r] = x.x;
         L.rb[r] = x.y;
     }
+    if (want_jac) {
+        float4 *A4 = reinterpret_cast<float4 *>(L.A);
+        const int n4 = (NP * S + 3) >> 2;
+        for (int q = t; q < n4; q += T) A4[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     __syncthreads();
     for (int i = t; i < NP; i += T) {
         float *row = L.A + i * S;
-        if (want_jac)
-            for (int j = 0; j < NP; j++) row[j] = (i == j) ? 1.0f : 0.0f;
+        if (want_jac) row[i] = 1.0f;
         float acc = 0.0f;
         if (i < N)
             for (int e = net.f_ptr[i]; e < net.f_ptr[i + 1]; e++) {
@@ -283,7 +369,7 @@ __device__ inline void dense_refine(const MassActionDevice &net, const float *yv
 }
 
 template <int METHOD, int PREC>
-__global__ void __launch_bounds__(256) ma_dense_integrate(IntegrateArgs a, MassActionDevice net, int ir_sweeps) {
+__global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassActionDevice net, int ir_sweeps) {
     extern __shared__ float lds[];
     const int N = net.N, t = threadIdx.x, T = blockDim.x;
     DenseLayout L;
@@ -302,9 +388,9 @@ __global__ void __launch_bounds__(256) ma_dense_integrate(IntegrateArgs a, MassA
     L.hdb = L.hda + R;
     L.ra = reinterpret_cast<int *>(L.hdb + R);
     L.rb = L.ra + R;
-    L.A = reinterpret_cast<float *>(L.rb + R);
+    L.A = lds + dense_a_offset(NP, R);
     __shared__ int s_flag, s_sid;
-    __shared__ double s_err[4];
+    __shared__ double s_err[8];
     unsigned long long steps = 0;
     unsigned int *head = reinterpret_cast<unsigned int *>(a.counters);
     for (;;) {
@@ -388,7 +474,7 @@ __global__ void __launch_bounds__(256) ma_dense_integrate(IntegrateArgs a, MassA
 // LDS bytes of ma_dense_integrate for a network.
 inline size_t dense_lds_bytes(int N, int R) {
     const int NP = (N + 15) & ~15;
-    return sizeof(float) * ((size_t)6 * NP + 5 * (size_t)R + (size_t)NP * (NP + 1));
+    return sizeof(float) * ((size_t)dense_a_offset(NP, R) + (((size_t)NP * (NP + 1) + 3) & ~(size_t)3));
 }
 
 }  // namespace wind

@@ -124,8 +124,8 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
     if (fast && (N + 15) / 16 * 16 <= 192) {
         const size_t lds = dense_lds_bytes(N, d.R);
         if (lds + 128 <= (size_t)max_lds) {
-            int T = ((N + 15) & ~15) <= 64 ? 64 : 256;
-            if (sc.block == 64 || sc.block == 128 || sc.block == 256) T = sc.block;
+            int T = ((N + 15) & ~15) <= 64 ? 64 : 512;
+            if (sc.block == 64 || sc.block == 128 || sc.block == 256 || sc.block == 512) T = sc.block;
             const SolverChoice &g = sc;
             int solver = sc.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : sc.solver;
             switch (solver) {

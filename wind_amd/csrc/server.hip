@@ -86,6 +86,9 @@ struct Slot {
     hipEvent_t done = nullptr;
     unsigned long long *counters = nullptr;  // device, 4 words
     unsigned long long *host_counters = nullptr;  // pinned, 4 words
+    char *stage = nullptr;  // pinned staging for a request's host arrays
+    size_t stage_bytes = 0;
+    size_t out_off = 0, out_bytes = 0;  // where the results sit in `stage`
     int64_t request = -1;
 };
 
@@ -141,24 +144,49 @@ size_t footprint(const Request &r) {
     return ((cb + 255) & ~(size_t)255) + (size_t)r.d.nsys * N * sizeof(float);
 }
 
+// Grows a slot's pinned staging buffer.  Host arrays go through it so that
+// copies are truly asynchronous and the scheduler never blocks on a slot.
+bool ensure_stage(Slot &s, size_t bytes) {
+    if (bytes <= s.stage_bytes) return true;
+    if (s.stage) {
+        (void)hipStreamSynchronize(s.stream);
+        (void)hipHostFree(s.stage);
+    }
+    s.stage = nullptr;
+    s.stage_bytes = 0;
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipHostMalloc((void **)&s.stage, want, 0) != hipSuccess) {
+        s.stage = nullptr;
+        return false;
+    }
+    s.stage_bytes = want;
+    return true;
+}
+
 // Enqueues an admitted request on its slot.  Called with srv->mu held.
 bool start(wind_server *srv, Request &r, int si) {
     Slot &s = srv->slots[si];
     const wind_network *net = r.d.block ? r.d.block->net : r.d.net;
     const int nsys = r.d.block ? r.d.block->nsys : r.d.nsys;
     const int stride = r.d.block ? r.d.block->const_stride : (r.d.const_stride > 0 ? r.d.const_stride : net->nconst);
+    const size_t cb = ((size_t)(nsys - 1) * stride + net->nconst) * sizeof(float);
+    const size_t cba = (cb + 255) & ~(size_t)255;
+    const size_t eb = (size_t)nsys * net->N * sizeof(float);
+    if ((r.d.equations || !r.d.block) && !ensure_stage(s, cba + eb)) {
+        r.error = "cannot allocate pinned staging memory";
+        return false;
+    }
     float *dc, *de;
     if (r.d.block) {
         dc = r.d.block->constants;
         de = r.d.block->equations;
         r.d.block->busy = true;
     } else {
-        const size_t cb = ((size_t)(nsys - 1) * stride + net->nconst) * sizeof(float);
         dc = (float *)(srv->arena_base + r.off);
-        de = (float *)(srv->arena_base + r.off + ((cb + 255) & ~(size_t)255));
-        if (hipMemcpyAsync(dc, r.d.constants, cb, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
-            hipMemcpyAsync(de, r.d.equations, (size_t)nsys * net->N * sizeof(float), hipMemcpyHostToDevice,
-                           s.stream) != hipSuccess) {
+        de = (float *)(srv->arena_base + r.off + cba);
+        std::memcpy(s.stage, r.d.constants, cb);
+        std::memcpy(s.stage + cba, r.d.equations, eb);
+        if (hipMemcpyAsync(dc, s.stage, cba + eb, hipMemcpyHostToDevice, s.stream) != hipSuccess) {
             r.error = "copy to device failed";
             return false;
         }
@@ -190,9 +218,9 @@ bool start(wind_server *srv, Request &r, int si) {
             return false;
         }
     }
-    if (r.d.equations)
-        (void)hipMemcpyAsync(r.d.equations, de, (size_t)nsys * net->N * sizeof(float), hipMemcpyDeviceToHost,
-                             s.stream);
+    s.out_off = cba;
+    s.out_bytes = r.d.equations ? eb : 0;
+    if (s.out_bytes) (void)hipMemcpyAsync(s.stage + cba, de, eb, hipMemcpyDeviceToHost, s.stream);
     (void)hipMemcpyAsync(s.host_counters, s.counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                          s.stream);
     (void)hipEventRecord(s.done, s.stream);
@@ -214,6 +242,7 @@ void scheduler(wind_server *srv) {
             if (s.request < 0) continue;
             if (hipEventQuery(s.done) != hipSuccess) continue;
             Request &r = srv->reqs[s.request];
+            if (s.out_bytes) std::memcpy(r.d.equations, s.stage + s.out_off, s.out_bytes);
             r.finished = Clock::now();
             r.steps = (int64_t)s.host_counters[1];
             r.state = DONE;
@@ -307,6 +336,7 @@ void wind_server_destroy(wind_server *srv) {
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.counters) (void)hipFree(s.counters);
         if (s.host_counters) (void)hipHostFree(s.host_counters);
+        if (s.stage) (void)hipHostFree(s.stage);
     }
     if (srv->arena_base) (void)hipFree(srv->arena_base);
     delete srv;

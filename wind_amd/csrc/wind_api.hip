@@ -396,12 +396,13 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     const int NP = (net->N + 15) & ~15;
     if (NP > 192) return v;
     v.clear();
-    const int solvers[2] = {WIND_SOLVER_MFMA_F32, WIND_SOLVER_VALU};
-    const int blocks[3] = {64, 128, 256};
-    for (int sv : solvers) {
-        if (o.solver != WIND_SOLVER_AUTO && sv != o.solver) continue;
+    // the tile products of the dense solver: FP32 MFMA unless the caller names one
+    const int sv = o.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : o.solver;
+    const int blocks[4] = {64, 128, 256, 512};
+    {
         for (int blk : blocks) {
             if (NP > 64 && blk == 64) continue;
+            if (NP <= 32 && blk > 128) continue;
             SolverChoice sc;
             sc.solver = sv;
             sc.ir_sweeps = o.ir_sweeps;

@@ -30,7 +30,7 @@ def _logu(rng, lo, hi, size=None):
 
 # --------------------------------------------------------------------- NR_test
 def nr_test_systems(nsys: int, seed: int = 0):
-    """Config 1: y0 = (1, 0), constants a*[998, 1998, -999, -1999], a ~ U[0.9, 1.1]."""
+    """NR_test workload: y0 = (1, 0), constants a*[998, 1998, -999, -1999], a ~ U[0.9, 1.1]."""
     rng = np.random.default_rng(seed)
     a = rng.uniform(0.9, 1.1, nsys).astype(np.float32)
     base = np.array([998.0, 1998.0, -999.0, -1999.0], dtype=np.float32)
@@ -49,7 +49,7 @@ def nr_test_exact(a, t):
 
 # --------------------------------------------------------------------- hhe6
 def hhe6_params(seed: int = HHE6_DEFAULT_SEED) -> np.ndarray:
-    """Network parameters [lnA[7], b[7], C[7], Gamma[3]] (config 2)."""
+    """Network parameters [lnA[7], b[7], C[7], Gamma[3]] (hhe6 workload)."""
     rng = np.random.default_rng(seed)
     A = _logu(rng, 1e-13, 1e-9, 7)
     b = rng.uniform(-0.8, 0.5, 7)
@@ -59,7 +59,7 @@ def hhe6_params(seed: int = HHE6_DEFAULT_SEED) -> np.ndarray:
 
 
 def hhe6_systems(nsys: int, seed: int = 1):
-    """Per-system initial state and constants (config 2).
+    """Per-system initial state and constants for the hhe6 workload.
 
     T ~ logU[1e2, 1e8] K, n_H ~ logU[1e-4, 1e4] cm^-3, He/H = 0.08.  The
     hydrogen ionised fraction is U[0, 1]; helium is split over He0/He+/He++
@@ -105,7 +105,7 @@ MIXED_WEIGHTS = (0.40, 0.25, 0.15, 0.12, 0.08)
 
 
 def mixed_population(nsys: int, seed: int):
-    """Config 4: N drawn from MIXED_SIZES with MIXED_WEIGHTS; returns the N of each system."""
+    """Mixed-N population: N drawn from MIXED_SIZES with MIXED_WEIGHTS; returns the N of each system."""
     rng = np.random.default_rng(seed)
     return rng.choice(np.array(MIXED_SIZES), size=nsys, p=np.array(MIXED_WEIGHTS))
 
@@ -114,7 +114,7 @@ MIXED_SCALE_RANGE = (0.1, 10.0)
 
 
 def mixed_workload(nsys: int, seed: int = 4):
-    """Config 4: buckets of random mass-action networks, one network per N in
+    """Mixed-N workload: buckets of random mass-action networks, one network per N in
     MIXED_SIZES (seeded by N), systems assigned to N with MIXED_WEIGHTS, per-
     system rate scales logU(MIXED_SCALE_RANGE) so step counts vary widely.
     Returns a list of dicts {N, rx, k, y0, scale, index} (index = positions
@@ -127,3 +127,19 @@ def mixed_workload(nsys: int, seed: int = 4):
         y0, sc = mass_action_systems(len(idx), N, seed=2000 + N + seed, scale_range=MIXED_SCALE_RANGE)
         out.append({"N": N, "rx": rx, "k": k, "y0": y0, "scale": sc, "index": idx})
     return out
+
+
+# ------------------------------------------------------------ dense large N
+DENSE_N = 157
+DENSE_NSYS = 65536
+DENSE_NET_SEED = 157
+DENSE_SYS_SEED = 1157
+
+
+def dense_workload(nsys: int = DENSE_NSYS):
+    """One random mass-action network with N = 157 species and R = 4N
+    reactions; y0 ~ logU[1e-12, 1], unit rate scales.  Returns (rx, k, y0, scale)."""
+    rx, k = mass_action_network(DENSE_N, seed=DENSE_NET_SEED)
+    y0, scale = mass_action_systems(nsys, DENSE_N, seed=DENSE_SYS_SEED)
+    return rx, k, y0, scale
+
