@@ -1,0 +1,270 @@
+"""GPU tests of the fast dense solver, the mixed-N batch and the request server.
+
+* Dense solver (fast mode, mass-action networks): on a fixed accepted-step
+  sequence its error against the oracle's FP64 run stays within a small
+  multiple of the oracle's own float32 error; with the adaptive controller it
+  meets the final-state criterion (relative error <= 1e-3 against FP64
+  Radau solutions, tests/golden/truth_mass_action.npz).
+* Batch: every system's result is bit-identical to integrating its bucket
+  alone (the kernels are deterministic and do not depend on grid or block
+  shape); hipGraph replays give the eager result on every replay.
+* Server: each request's result is bit-identical alone and in the mix;
+  resident blocks advance in order; a small HBM budget throttles admission
+  without changing results.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from wind_amd import api
+from wind_amd import networks as W
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TRUTH = os.path.join(ROOT, "tests", "golden", "truth_mass_action.npz")
+# Controller tolerances of the mass-action workloads: a small absolute floor
+# (species span 1e-12..1) so that trace species are controlled relatively.
+MA_ABS, MA_REL = 1e-10, 1e-4
+
+
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def rel_err(y, yt, floor=1e-6):
+    """Per-system max over species of |y - yt| / max(|yt|, floor * sum|yt|)."""
+    y = np.asarray(y, np.float64)
+    yt = np.asarray(yt, np.float64)
+    tot = np.abs(yt).sum(axis=1, keepdims=True)
+    e = np.abs(y - yt) / np.maximum(np.abs(yt), floor * tot)
+    return np.where(np.isnan(e), np.inf, e).max(axis=1)
+
+
+@pytest.mark.parametrize("N", [10, 32, 64, 157])
+def test_dense_fixed_sequence_vs_fp64(N):
+    """Huge tolerances accept every attempt: the step sequence is fixed
+    (h0 = tend/4096, growing 4x).  The fast GPU result is compared with the
+    oracle's FP64 run of the same sequence; its largest and its median error
+    over the systems must stay within 4x those of the oracle's float32 run
+    (plus 1e-6): two float32 solvers with different rounding, compared by
+    the size of their errors rather than system by system."""
+    ns = 32
+    rx, k = W.mass_action_network(N, seed=N)
+    y0, sc = W.mass_action_systems(ns, N, seed=900 + N, scale_range=(0.5, 2.0))
+    onet = O.mass_action(N, rx, k)
+    y64 = y0.astype(np.float64)
+    c64 = sc.astype(np.float64)
+    O.gold_integrate_f64(0, onet, 0.0, 1e9, 4096, c64, y64, ns, 1e30, 1e30, const_stride=1, controller=1)
+    y32 = y0.copy()
+    O.gold_integrate(0, onet, 0.0, 1e9, 4096, sc, y32, ns, 1e30, 1e30, const_stride=1, controller=1)
+    y = y0.copy()
+    steps = np.zeros(ns, np.int32)
+    api.integrate(api.Network.mass_action(N, rx, k), 0.0, 1e9, 4096, sc, y, ns, 1e30, 1e30, mode=api.MODE_FAST,
+                  controller=api.CTRL_ADAPTIVE, steps_out=steps)
+    ref = y64.reshape(ns, N)
+    e_gpu = rel_err(y.reshape(ns, N), ref)
+    e_cpu = rel_err(y32.reshape(ns, N), ref)
+    print(f"N {N}: gpu max {e_gpu.max():.2e} median {np.median(e_gpu):.2e}; "
+          f"oracle float32 max {e_cpu.max():.2e} median {np.median(e_cpu):.2e}")
+    assert (steps == steps[0]).all()
+    assert e_gpu.max() <= 4.0 * e_cpu.max() + 1e-6
+    assert np.median(e_gpu) <= 4.0 * np.median(e_cpu) + 1e-6
+
+
+def _truth():
+    return np.load(TRUTH)
+
+
+@pytest.mark.parametrize("N", list(W.MIXED_SIZES))
+def test_dense_accuracy_vs_fp64_truth(N):
+    """Mixed-N workload samples: fast solver + adaptive controller at
+    ABS 1e-10, REL 1e-4; at least 90% of systems within 1e-3 of the FP64
+    truth (floor 1e-6 of the system's total abundance)."""
+    T = _truth()
+    b = [b for b in W.mixed_workload(int(T["mixed_nsys"]), seed=int(T["mixed_seed"])) if b["N"] == N][0]
+    sel = T[f"mixed_{N}_idx"]
+    y = b["y0"].reshape(-1, N)[sel].reshape(-1).copy()
+    c = b["scale"][sel].copy()
+    st = np.zeros(len(sel), np.int32)
+    api.integrate(api.Network.mass_action(N, b["rx"], b["k"]), 0.0, float(T["tend"]), 1, c, y, len(sel), MA_ABS,
+                  MA_REL, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE, status_out=st, max_attempts=100000)
+    e = rel_err(y.reshape(-1, N), T[f"mixed_{N}_y"])
+    print(f"N {N}: frac<=1e-3 {np.mean(e <= 1e-3):.3f} median {np.median(e):.2e} max {e.max():.2e}")
+    assert (st == 0).all()
+    assert np.mean(e <= 1e-3) >= 0.9
+
+
+def test_dense_157_accuracy_vs_fp64_truth():
+    T = _truth()
+    rx, k, y0, sc = W.dense_workload(int(T["dense_nsys"]))
+    sel = T["dense_157_idx"]
+    N = W.DENSE_N
+    y = y0.reshape(-1, N)[sel].reshape(-1).copy()
+    c = sc[sel].copy()
+    st = np.zeros(len(sel), np.int32)
+    api.integrate(api.Network.mass_action(N, rx, k), 0.0, float(T["tend"]), 1, c, y, len(sel), MA_ABS, MA_REL,
+                  mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE, status_out=st, max_attempts=100000)
+    e = rel_err(y.reshape(-1, N), T["dense_157_y"])
+    print(f"N 157: frac<=1e-3 {np.mean(e <= 1e-3):.3f} median {np.median(e):.2e} max {e.max():.2e}")
+    assert (st == 0).all()
+    assert np.mean(e <= 1e-3) >= 0.9
+
+
+@pytest.mark.parametrize("solver,ir", [(api.SOLVER_MFMA_BF16, 2), (api.SOLVER_MFMA_BF16X3, 1)])
+def test_dense_bf16_with_refinement(solver, ir):
+    """BF16-input MFMA trailing updates followed by refinement sweeps with an
+    FP64 residual: same fixed sequence as above at N = 157, largest error
+    within 10x the FP32 solver's (plus 1e-5)."""
+    N, ns = 157, 16
+    rx, k = W.mass_action_network(N, seed=N)
+    y0, sc = W.mass_action_systems(ns, N, seed=900 + N, scale_range=(0.5, 2.0))
+    y64 = y0.astype(np.float64)
+    O.gold_integrate_f64(0, O.mass_action(N, rx, k), 0.0, 1e9, 4096, sc.astype(np.float64), y64, ns, 1e30, 1e30,
+                         const_stride=1, controller=1)
+    net = api.Network.mass_action(N, rx, k)
+    out = {}
+    for name, s, r in (("f32", api.SOLVER_MFMA_F32, 0), ("test", solver, ir)):
+        y = y0.copy()
+        api.integrate(net, 0.0, 1e9, 4096, sc, y, ns, 1e30, 1e30, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE,
+                      solver=s, ir_sweeps=r)
+        out[name] = rel_err(y.reshape(ns, N), y64.reshape(ns, N))
+    print("f32 max", out["f32"].max(), "bf16 variant max", out["test"].max())
+    assert out["test"].max() <= 10.0 * out["f32"].max() + 1e-5
+
+
+def _mixed_buckets(torch, nsys, seed=4):
+    dev = torch.device("cuda:0")
+    bk = []
+    for b in W.mixed_workload(nsys, seed=seed):
+        n = len(b["scale"])
+        if n == 0:
+            continue
+        y0 = torch.from_numpy(b["y0"]).to(dev)
+        bk.append(dict(net=api.Network.mass_action(b["N"], b["rx"], b["k"]), N=b["N"], nsys=n, y0=y0,
+                       equations=y0.clone(), constants=torch.from_numpy(b["scale"]).to(dev),
+                       steps_out=torch.zeros(n, dtype=torch.int32, device=dev),
+                       status_out=torch.zeros(n, dtype=torch.int32, device=dev)))
+    return bk
+
+
+def test_batch_matches_buckets_alone():
+    torch = torch_cuda()
+    bk = _mixed_buckets(torch, 4096)
+    tend = W.MASS_ACTION_TEND
+    alone = []
+    for b in bk:
+        y = b["y0"].clone()
+        api.integrate(b["net"], 0.0, tend, 1, b["constants"], y, b["nsys"], MA_ABS, MA_REL, mode=api.MODE_FAST,
+                      controller=api.CTRL_ADAPTIVE, device_pointers=True, max_attempts=100000)
+        alone.append(y.cpu().numpy())
+    total, ms = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, max_attempts=100000)
+    torch.cuda.synchronize()
+    steps = sum(int(b["steps_out"].sum()) for b in bk)
+    print("batch ms", ms, "substeps", total)
+    assert total == steps
+    for b, ya in zip(bk, alone):
+        assert np.array_equal(b["equations"].cpu().numpy(), ya), b["N"]
+        assert (b["status_out"].cpu().numpy() == 0).all()
+
+
+def test_batch_graph_replays():
+    """Captured batch replayed three times, with eager work and host
+    synchronisation between replays; every replay equals the eager batch."""
+    torch = torch_cuda()
+    bk = _mixed_buckets(torch, 2048, seed=7)
+    tend = W.MASS_ACTION_TEND
+    eager_total, _ = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, max_attempts=100000)
+    eager = [b["equations"].cpu().numpy() for b in bk]
+    for rep in range(3):
+        for b in bk:
+            b["equations"].copy_(b["y0"])
+        junk = torch.ones(1 << 16, device="cuda") * rep  # eager work between replays
+        torch.cuda.synchronize()
+        total, _ = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=True, max_attempts=100000)
+        torch.cuda.synchronize()
+        assert float(junk.sum()) == float(rep * (1 << 16))
+        assert total == eager_total, rep
+        for b, ye in zip(bk, eager):
+            assert np.array_equal(b["equations"].cpu().numpy(), ye), (rep, b["N"])
+
+
+def _requests(seed=3, n=12):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    hp = W.hhe6_params()
+    hnet = api.Network.builtin(api.NET_HHE6, hp)
+    ma = {}
+    for i in range(n):
+        if i % 3 == 2:
+            N = int(rng.choice([10, 64, 157]))
+            if N not in ma:
+                rx, k = W.mass_action_network(N, seed=1000 + N)
+                ma[N] = api.Network.mass_action(N, rx, k)
+            nsys = int(rng.integers(64, 512))
+            y0, c = W.mass_action_systems(nsys, N, seed=50 + i, scale_range=W.MIXED_SCALE_RANGE)
+            reqs.append(dict(net=ma[N], y0=y0, c=c, nsys=nsys, tend=W.MASS_ACTION_TEND, abs=MA_ABS, rel=MA_REL))
+        else:
+            nsys = int(np.exp(rng.uniform(np.log(1000), np.log(40000))))
+            y0, c = W.hhe6_systems(nsys, seed=100 + i)
+            reqs.append(dict(net=hnet, y0=y0, c=c, nsys=nsys, tend=float(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))),
+                             abs=1e-5, rel=1e-5))
+    return reqs
+
+
+def _alone(r):
+    y = r["y0"].copy()
+    api.integrate(r["net"], 0.0, r["tend"], 1, r["c"], y, r["nsys"], r["abs"], r["rel"], mode=api.MODE_FAST,
+                  controller=api.CTRL_ADAPTIVE)
+    return y
+
+
+@pytest.mark.parametrize("budget_mib,nslots", [(1024, 4), (4, 2)])
+def test_server_results_same_alone_and_in_mix(budget_mib, nslots):
+    reqs = _requests()
+    alone = [_alone(r) for r in reqs]
+    srv = api.Server(budget_mib << 20, nslots=nslots)
+    outs, ids = [], []
+    for r in reqs:
+        if budget_mib == 4 and r["nsys"] * r["net"].N * 4 * 2 > (4 << 20):
+            continue
+        y = r["y0"].copy()
+        outs.append((y, len(ids)))
+        ids.append(srv.submit(0.0, r["tend"], r["abs"], r["rel"], net=r["net"], constants=r["c"], equations=y,
+                              nsys=r["nsys"]))
+    running, pending, used = srv.inflight()
+    lat = [srv.wait(i)[0] for i in ids]
+    srv.close()
+    print("requests", len(ids), "in flight at submit end", running, pending, "p50 ms", np.percentile(lat, 50) / 1e3)
+    k = 0
+    for r, ya in zip(reqs, alone):
+        if budget_mib == 4 and r["nsys"] * r["net"].N * 4 * 2 > (4 << 20):
+            continue
+        assert np.array_equal(outs[k][0], ya), k
+        k += 1
+    assert k == len(ids) and k >= 4
+
+
+def test_server_resident_block_advances_in_order():
+    hp = W.hhe6_params()
+    net = api.Network.builtin(api.NET_HHE6, hp)
+    n = 3000
+    y0, c = W.hhe6_systems(n, seed=11)
+    # reference: three consecutive integrations of the same state
+    y = y0.copy()
+    for (t0, t1) in ((0.0, 1e3), (1e3, 1e4), (1e4, 1e5)):
+        api.integrate(net, t0, t1, 1, c, y, n, 1e-5, 1e-5, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE)
+    srv = api.Server(64 << 20, nslots=3)
+    blk = srv.block(net, c, y0, n)
+    ids = [srv.submit(t0, t1, 1e-5, 1e-5, block=blk) for (t0, t1) in ((0.0, 1e3), (1e3, 1e4), (1e4, 1e5))]
+    for i in ids:
+        srv.wait(i)
+    out = np.empty_like(y0)
+    srv.read_block(blk, out)
+    srv.close()
+    assert np.array_equal(out, y)

@@ -69,6 +69,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise WindError(f"wind_amd native library not built: {path} (run python -m wind_amd.build)")
+    # PyTorch ships its own HIP runtime under the same soname.  Loading torch
+    # first makes this library bind to that one runtime, so device pointers
+    # and streams from torch and from this library live in one HIP context.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     lib.wind_network_create_builtin.argtypes = [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
     lib.wind_network_create_mass_action.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
