@@ -17,7 +17,11 @@ reference algorithm as the reference runs it (exact arithmetic, reference
 controller, tol 1e-3) with its own accuracy, the one-workgroup-per-system
 kernel with the headline's algorithm on a subset (the 5x target's
 denominator), and the oracle (CPU restatement of the reference c_baseline)
-timed on this host's cores.
+timed on this host's cores.  Beside the headline: the mixed-N mass-action
+workload (2^18 systems, N in {5, 10, 32, 64, 157}) in one autotuned,
+graph-captured batch with its accuracy against FP64 and the plain
+one-workgroup-per-system kernel's rate; the N = 157 dense kernel against the
+FP32 MFMA peak; and the request server under Poisson arrivals (p50/p99).
 
     python bench.py [--steps K] [--warmup W] [--mode fast|exact] [--gpus 1]
 """
@@ -97,6 +101,200 @@ def cpu_baseline(params, y0, nH, seconds, tol, controller):
             "algorithm": ("reference controller" if controller == 0 else "adaptive controller") + f", tol {tol:g}"}
 
 
+# ----------------------------------------------------------- mixed-N batch
+MIXED_NSYS = 1 << 18
+MA_ABS, MA_REL = 1e-10, 1e-4     # mass-action controller tolerances
+PLAIN_SAMPLE = {5: 8192, 10: 8192, 32: 4096, 64: 2048, 157: 256}
+
+
+def dense_flops(N):
+    """Algorithmic flops per sub-step of a mass-action system (counted once):
+    RHS + Jacobian 64N, form I - hJ 2N^2, LU 2/3 N^3, two triangular solves
+    2N^2, update 2N."""
+    return 64 * N + 2 * N * N + (2 * N**3) // 3 + 2 * N * N + 2 * N
+
+
+def ma_rel_err(y, yt, floor=1e-6):
+    tot = np.abs(yt).sum(axis=1, keepdims=True)
+    e = np.abs(y - yt) / np.maximum(np.abs(yt), floor * tot)
+    return np.where(np.isnan(e), np.inf, e).max(axis=1)
+
+
+def mixed_batch(torch, api, W, nsys, steps, plain):
+    """Mixed-N mass-action workload in one batched, autotuned, graph-captured
+    call; its accuracy on the FP64-truth sample; the plain one-workgroup-per-
+    system kernel (exact arithmetic, same controller and tolerances) timed
+    per bucket on samples."""
+    dev = torch.device("cuda:0")
+    wl = W.mixed_workload(nsys, seed=4)
+    bk = []
+    for b in wl:
+        n = len(b["scale"])
+        y0 = torch.from_numpy(b["y0"]).to(dev)
+        bk.append(dict(net=api.Network.mass_action(b["N"], b["rx"], b["k"]), N=b["N"], nsys=n, y0=y0,
+                       equations=y0.clone(), constants=torch.from_numpy(b["scale"]).to(dev),
+                       steps_out=torch.zeros(n, dtype=torch.int32, device=dev),
+                       status_out=torch.zeros(n, dtype=torch.int32, device=dev)))
+    tend = W.MASS_ACTION_TEND
+    t = time.perf_counter()
+    api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=True, max_attempts=100000)  # tune + capture
+    torch.cuda.synchronize()
+    warm_s = time.perf_counter() - t
+    times = []
+    for _ in range(steps):
+        for b in bk:
+            b["equations"].copy_(b["y0"])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        total, _ = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=True, max_attempts=100000)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    sec = float(np.median(times))
+    out = {"workload": f"mixed-N mass-action, {nsys} systems, N in {list(W.MIXED_SIZES)} with weights "
+                       f"{list(W.MIXED_WEIGHTS)}, rate scales logU[0.1, 10], t in [0, {tend:g}], SIE, adaptive "
+                       f"controller abs {MA_ABS:g} rel {MA_REL:g}",
+           "value": nsys / sec, "unit": "systems/s", "ms_per_step": sec * 1e3, "steps": steps,
+           "first_call_s_incl_autotune_and_capture": warm_s, "substeps": int(total), "buckets": {}}
+    tp = os.path.join(ROOT, "tests", "golden", "truth_mass_action.npz")
+    T = np.load(tp) if os.path.exists(tp) else None
+    fracs = []
+    for b in bk:
+        st = b["status_out"].cpu().numpy()
+        sp = b["steps_out"].cpu().numpy().astype(np.int64)
+        info = {"nsys": b["nsys"], "substeps_mean": float(sp.mean()), "substeps_p90": float(np.quantile(sp, 0.9)),
+                "status_ok": int(np.sum(st == 0))}
+        if T is not None and nsys == int(T["mixed_nsys"]):
+            sel = T[f"mixed_{b['N']}_idx"]
+            y = b["equations"].cpu().numpy().reshape(-1, b["N"])[sel].astype(np.float64)
+            e = ma_rel_err(y, T[f"mixed_{b['N']}_y"])
+            info["frac_rel_err_le_1e-3"] = float(np.mean(e <= 1e-3))
+            info["median_rel_err"] = float(np.median(e))
+            fracs.append(e)
+        out["buckets"][str(b["N"])] = info
+    if fracs:
+        e = np.concatenate(fracs)
+        out["accuracy"] = {"sample": int(len(e)), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
+                           "median_rel_err": float(np.median(e)),
+                           "error_definition": "max over species |y - y_fp64| / max(|y_fp64|, 1e-6 sum|y_fp64|)"}
+    if plain:
+        est = 0.0
+        rates = {}
+        for b in bk:
+            n = min(b["nsys"], PLAIN_SAMPLE.get(b["N"], 1024))
+            y = b["y0"][: n * b["N"]].clone()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            api.integrate(b["net"], 0.0, tend, 1, b["constants"][:n], y, n, MA_ABS, MA_REL, mode=api.MODE_EXACT,
+                          kernel=api.KERNEL_BLOCK, device_pointers=True, controller=api.CTRL_ADAPTIVE,
+                          max_attempts=100000)
+            torch.cuda.synchronize()
+            rates[str(b["N"])] = n / (time.perf_counter() - t)
+            est += b["nsys"] / rates[str(b["N"])]
+        out["plain_gpu"] = {"value": nsys / est, "unit": "systems/s",
+                            "kernel": "ma_block_integrate: one workgroup per system, equation per thread, "
+                                      "Gauss-Jordan inverse (the reference's algorithm), exact arithmetic, same "
+                                      "controller and tolerances",
+                            "method": "per-bucket rates on the first systems of each bucket "
+                                      f"({PLAIN_SAMPLE}), summed over the bucket sizes",
+                            "bucket_rates": rates, "speedup": (nsys / sec) / (nsys / est)}
+    return out
+
+
+def dense_roofline(torch, api, W, nsys):
+    """The N = 157 dense workload alone on a sample: algorithmic flops per
+    second of the dense kernel against the FP32 MFMA peak."""
+    dev = torch.device("cuda:0")
+    rx, k, y0, sc = W.dense_workload(nsys)
+    net = api.Network.mass_action(W.DENSE_N, rx, k)
+    y = torch.from_numpy(y0).to(dev)
+    c = torch.from_numpy(sc).to(dev)
+    tot, kms = api.integrate(net, 0.0, W.MASS_ACTION_TEND, 1, c, y, nsys, MA_ABS, MA_REL, mode=api.MODE_FAST,
+                             device_pointers=True, controller=api.CTRL_ADAPTIVE, want_kernel_ms=True,
+                             max_attempts=100000)
+    F = dense_flops(W.DENSE_N)
+    ach = F * float(tot) / (kms / 1e3) / 1e12
+    return {"workload": f"N = 157 mass-action, first {nsys} of {W.DENSE_NSYS} systems, t in [0, 1e9]",
+            "kernel": "ma_dense_integrate<SIE, MFMA_F32>", "systems_per_s": nsys / (kms / 1e3),
+            "kernel_ms": kms, "substeps": int(tot), "flops_per_substep": F,
+            "us_per_substep_per_cu": kms * 1e3 / (float(tot) / 256.0),
+            "bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_FP32_TFLOPS}
+
+
+# ------------------------------------------------------------------ serving
+def serving(api, W, seconds, load_frac, seed=5):
+    """Continuous-batching request server: resident cell blocks of mixed
+    networks; Poisson arrivals at `load_frac` of the capacity measured with a
+    closed-loop burst; each request advances one block by dt and returns its
+    state to the host."""
+    rng = np.random.default_rng(seed)
+    hp = W.hhe6_params()
+    nets = {"hhe6": api.Network.builtin(api.NET_HHE6, hp)}
+    for b in W.mixed_workload(1 << 12, seed=4):
+        nets[b["N"]] = (api.Network.mass_action(b["N"], b["rx"], b["k"]), b)
+    srv = api.Server(8 << 30, nslots=8)
+    blocks = []
+    for i in range(24):
+        if i % 3 != 2:
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(256e3))))
+            y0, c = W.hhe6_systems(n, seed=200 + i)
+            blocks.append(dict(kind="hhe6", n=n, N=6, h=srv.block(nets["hhe6"], c, y0, n), out=np.empty_like(y0),
+                               dt=lambda: float(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))), tol=(1e-5, 1e-5)))
+        else:
+            N = int(rng.choice(np.array(W.MIXED_SIZES), p=np.array(W.MIXED_WEIGHTS)))
+            net = nets[N][0]
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(256e3))) / max(1.0, (N / 10.0) ** 2))
+            n = max(n, 64)
+            y0, c = W.mass_action_systems(n, N, seed=300 + i, scale_range=W.MIXED_SCALE_RANGE)
+            blocks.append(dict(kind=f"mass_action_{N}", n=n, N=N, h=srv.block(net, c, y0, n), out=np.empty_like(y0),
+                               dt=lambda: float(np.exp(rng.uniform(np.log(1e6), np.log(1e9)))), tol=(MA_ABS, MA_REL)))
+    clock = {id(b): 0.0 for b in blocks}
+
+    def submit(b):
+        t0 = clock[id(b)]
+        t1 = t0 + b["dt"]()
+        clock[id(b)] = t1
+        return srv.submit(t0, t1, b["tol"][0], b["tol"][1], block=b["h"], equations=b["out"])
+
+    # capacity: closed-loop burst, one request per block
+    t = time.perf_counter()
+    ids = [submit(b) for b in blocks]
+    for i in ids:
+        srv.wait(i)
+    burst_s = time.perf_counter() - t
+    sys_per_req = float(np.mean([b["n"] for b in blocks]))
+    cap_req_s = len(ids) / burst_s
+    rate = load_frac * cap_req_s
+    # open loop: Poisson arrivals for `seconds`
+    lat, served, pending = [], 0, []
+    t_start = time.perf_counter()
+    t_next = t_start
+    while True:
+        now = time.perf_counter()
+        if now - t_start >= seconds:
+            break
+        if now >= t_next:
+            b = blocks[int(rng.integers(len(blocks)))]
+            pending.append((submit(b), b["n"]))
+            t_next += rng.exponential(1.0 / rate)
+        else:
+            time.sleep(min(0.0005, t_next - now))
+    for rid, n in pending:
+        l_us, _ = srv.wait(rid)
+        lat.append(l_us / 1e3)
+        served += n
+    wall = time.perf_counter() - t_start
+    srv.close()
+    lat = np.array(lat)
+    return {"workload": "24 resident cell blocks (2/3 hhe6 with logU[1k, 256k] systems and dt logU[1e3, 1e6] yr; "
+                        "1/3 mixed-N mass-action with logU[1k, 256k]/max(1, (N/10)^2) systems and dt logU[1e6, 1e9]); "
+                        "each request advances one block and copies its state to the host",
+            "slots": 8, "capacity_requests_per_s": cap_req_s, "offered_load": load_frac,
+            "requests": int(len(lat)), "systems_per_s": served / wall, "mean_systems_per_request": sys_per_req,
+            "latency_ms": {"p50": float(np.quantile(lat, 0.5)), "p99": float(np.quantile(lat, 0.99)),
+                           "max": float(lat.max())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +307,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--mixed-nsys", type=int, default=MIXED_NSYS)
+    ap.add_argument("--mixed-steps", type=int, default=2)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--dense-nsys", type=int, default=4096)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested): systems are independent, so "
@@ -212,6 +417,9 @@ def main():
                            "launch shape), same controller and tolerance as the headline",
                  "speedup": value / pv}
 
+    mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain)
+    dense = None if a.no_mixed else dense_roofline(torch, api, W, a.dense_nsys)
+    serve = None if a.no_serve else serving(api, W, a.serve_seconds, a.serve_load)
     cpu = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, REF_TOL, 0)
     cpu_same = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, tol, ctrl)
 
@@ -229,6 +437,7 @@ def main():
         "status": {"ok": int(np.sum(st == 0)), "stalled": int(np.sum(st == 1)), "exhausted": int(np.sum(st == 2))},
         "accuracy": acc, "roofline": roofline, "reference_algorithm": ref_alg, "plain_gpu": plain,
         "cpu_baseline": cpu, "cpu_same_algorithm": cpu_same,
+        "mixed_batch": mixed, "dense_157": dense, "serving": serve,
     }
     print(json.dumps(line))
 
