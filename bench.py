@@ -314,6 +314,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--dense-nsys", type=int, default=4096)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device to device three times (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested): systems are independent, so "
@@ -327,6 +329,14 @@ def main():
     from wind_amd import networks as W
 
     dev = torch.device("cuda:0")
+    if a.copy_probe:
+        src = torch.ones(a.copy_probe // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.copy_probe}))
+        return
     params = W.hhe6_params()
     y0_h, nH_h = W.hhe6_systems(NSYS, seed=SEED)
     nsys = a.nsys

@@ -287,15 +287,52 @@ __device__ inline void dense_lu_solve(const float *A, int NP, int S, const float
     __syncthreads();
 }
 
+// LDS layout of one workgroup: per-system vectors, per-reaction scratch,
+// the network tables (copied once per workgroup) and the matrix.
 struct DenseLayout {
-    float *y, *yw, *y1, *f, *x, *r, *A, *w, *hda, *hdb;
-    int *ra, *rb;
+    float *y, *yw, *y1, *f, *x, *r, *A, *w, *hda, *hdb, *k;
+    int4 *rx;
+    int *fptr, *fent;
     int NP, S;
 };
 
 __device__ inline int dense_np(int N) { return (N + 15) & ~15; }
-// Float offset of the matrix in the LDS layout (16-byte aligned).
-__host__ __device__ inline int dense_a_offset(int NP, int R) { return (6 * NP + 5 * R + 3) & ~3; }
+// Float offsets of the LDS layout (int4 tables and the matrix 16-byte aligned).
+__host__ __device__ inline int dense_rx_offset(int NP, int R) { return (6 * NP + 4 * R + 3) & ~3; }
+__host__ __device__ inline int dense_a_offset(int NP, int R) {
+    return (dense_rx_offset(NP, R) + 4 * R + NP + 1 + 4 * R + 3) & ~3;
+}
+
+// Carves the LDS layout and copies the network tables into it (called by
+// the whole workgroup once; ends with a barrier).
+__device__ inline void dense_layout(float *lds, const MassActionDevice &net, DenseLayout &L) {
+    const int N = net.N, R = net.R, T = blockDim.x, t = threadIdx.x;
+    L.NP = dense_np(N);
+    L.S = L.NP + 1;
+    const int NP = L.NP;
+    L.y = lds;
+    L.yw = L.y + NP;
+    L.y1 = L.yw + NP;
+    L.f = L.y1 + NP;
+    L.x = L.f + NP;
+    L.r = L.x + NP;
+    L.w = L.r + NP;
+    L.hda = L.w + R;
+    L.hdb = L.hda + R;
+    L.k = L.hdb + R;
+    L.rx = reinterpret_cast<int4 *>(lds + dense_rx_offset(NP, R));
+    L.fptr = reinterpret_cast<int *>(L.rx + R);
+    L.fent = L.fptr + NP + 1;
+    L.A = lds + dense_a_offset(NP, R);
+    for (int r = t; r < R; r += T) {
+        L.rx[r] = reinterpret_cast<const int4 *>(net.rx)[r];
+        L.k[r] = net.k[r];
+    }
+    for (int i = t; i <= N; i += T) L.fptr[i] = net.f_ptr[i];
+    for (int i = N + 1 + t; i <= NP; i += T) L.fptr[i] = net.f_ptr[N];
+    for (int e = t; e < 4 * R; e += T) L.fent[e] = net.f_ent[e];
+    __syncthreads();
+}
 
 // f = rhs(yv) and A = I - hJ (dense, padded).  Phase 1: one thread per
 // reaction stores w = k s y_a y_b, h*da = h k s y_b, h*db = h k s y_a and the
@@ -307,14 +344,12 @@ __device__ inline void dense_eval(const MassActionDevice &net, const float *yv, 
                                   float hs, bool want_jac) {
     const int N = net.N, R = net.R, T = blockDim.x, t = threadIdx.x, NP = L.NP, S = L.S;
     for (int r = t; r < R; r += T) {
-        const int4 x = reinterpret_cast<const int4 *>(net.rx)[r];
-        const float ks = net.k[r] * scale;
+        const int4 x = L.rx[r];
+        const float ks = L.k[r] * scale;
         const float ya = yv[x.x], yb = yv[x.y];
         L.w[r] = (ks * ya) * yb;
         L.hda[r] = hs * (ks * yb);
         L.hdb[r] = hs * (ks * ya);
-        L.ra[r] = x.x;
-        L.rb[r] = x.y;
     }
     if (want_jac) {
         float4 *A4 = reinterpret_cast<float4 *>(L.A);
@@ -322,23 +357,35 @@ __device__ inline void dense_eval(const MassActionDevice &net, const float *yv, 
         for (int q = t; q < n4; q += T) A4[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();
+    // The tables and per-reaction values are read-only here and disjoint from
+    // the matrix rows: restrict lets the compiler issue their loads ahead of
+    // the row updates.
+    const int *__restrict__ fptr = L.fptr;
+    const int *__restrict__ fent = L.fent;
+    const int4 *__restrict__ rxs = L.rx;
+    const float *__restrict__ w = L.w;
+    const float *__restrict__ hda = L.hda;
+    const float *__restrict__ hdb = L.hdb;
     for (int i = t; i < NP; i += T) {
-        float *row = L.A + i * S;
+        float *__restrict__ row = L.A + i * S;
         if (want_jac) row[i] = 1.0f;
         float acc = 0.0f;
-        if (i < N)
-            for (int e = net.f_ptr[i]; e < net.f_ptr[i + 1]; e++) {
-                const int ent = net.f_ent[e];
+        if (i < N) {
+            const int e1 = fptr[i + 1];
+            for (int e = fptr[i]; e < e1; e++) {
+                const int ent = fent[e];
                 const int r = ent >> 1;
                 const bool neg = ent & 1;
-                acc = neg ? acc - L.w[r] : acc + L.w[r];
+                acc = neg ? acc - w[r] : acc + w[r];
                 if (want_jac) {
                     // J[i][a] += sign * da, J[i][b] += sign * db; A = I - hJ
-                    const float ha = L.hda[r], hb = L.hdb[r];
-                    row[L.ra[r]] = neg ? row[L.ra[r]] + ha : row[L.ra[r]] - ha;
-                    row[L.rb[r]] = neg ? row[L.rb[r]] + hb : row[L.rb[r]] - hb;
+                    const int4 x = rxs[r];
+                    const float ha = hda[r], hb = hdb[r];
+                    row[x.x] = neg ? row[x.x] + ha : row[x.x] - ha;
+                    row[x.y] = neg ? row[x.y] + hb : row[x.y] - hb;
                 }
             }
+        }
         L.f[i] = acc;
     }
     __syncthreads();
@@ -352,11 +399,11 @@ __device__ inline void dense_refine(const MassActionDevice &net, const float *yv
     for (int i = t; i < NP; i += T) {
         double jx = 0.0;
         if (i < N)
-            for (int e = net.f_ptr[i]; e < net.f_ptr[i + 1]; e++) {
-                const int ent = net.f_ent[e];
+            for (int e = L.fptr[i]; e < L.fptr[i + 1]; e++) {
+                const int ent = L.fent[e];
                 const int r = ent >> 1;
-                const int a = L.ra[r], b = L.rb[r];
-                const double ks = (double)(net.k[r] * scale);
+                const int a = L.rx[r].x, b = L.rx[r].y;
+                const double ks = (double)(L.k[r] * scale);
                 const double g = ks * ((double)yv[b] * (double)L.x[a] + (double)yv[a] * (double)L.x[b]);
                 jx = (ent & 1) ? jx - g : jx + g;
             }
@@ -373,22 +420,8 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
     extern __shared__ float lds[];
     const int N = net.N, t = threadIdx.x, T = blockDim.x;
     DenseLayout L;
-    L.NP = dense_np(N);
-    L.S = L.NP + 1;
+    dense_layout(lds, net, L);
     const int NP = L.NP;
-    const int R = net.R;
-    L.y = lds;
-    L.yw = L.y + NP;
-    L.y1 = L.yw + NP;
-    L.f = L.y1 + NP;
-    L.x = L.f + NP;
-    L.r = L.x + NP;
-    L.w = L.r + NP;
-    L.hda = L.w + R;
-    L.hdb = L.hda + R;
-    L.ra = reinterpret_cast<int *>(L.hdb + R);
-    L.rb = L.ra + R;
-    L.A = lds + dense_a_offset(NP, R);
     __shared__ int s_flag, s_sid;
     __shared__ double s_err[8];
     unsigned long long steps = 0;
