@@ -132,6 +132,7 @@ struct wind_server {
     int64_t next_id = 1;
     int num_cu = 0;
     std::string last_error;
+    std::vector<wind_block *> blocks;  // live resident blocks (freed by destroy)
 };
 
 namespace {
@@ -338,6 +339,7 @@ void wind_server_destroy(wind_server *srv) {
         if (s.host_counters) (void)hipHostFree(s.host_counters);
         if (s.stage) (void)hipHostFree(s.stage);
     }
+    for (wind_block *b : srv->blocks) delete b;
     if (srv->arena_base) (void)hipFree(srv->arena_base);
     delete srv;
 }
@@ -370,6 +372,10 @@ int wind_server_block_create(wind_server *srv, const wind_network *net, const fl
         delete b;
         return sfail(srv, WIND_ERR_DEVICE, "block upload failed");
     }
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->blocks.push_back(b);
+    }
     *out = b;
     return WIND_OK;
 }
@@ -393,8 +399,15 @@ int wind_server_block_read(wind_server *srv, wind_block *b, float *equations) {
 void wind_server_block_destroy(wind_server *srv, wind_block *b) {
     if (!srv || !b) return;
     std::unique_lock<std::mutex> lk(srv->mu);
-    srv->cv_done.wait(lk, [&] { return !b->busy; });
+    // wait until no running or queued request refers to the block
+    srv->cv_done.wait(lk, [&] {
+        if (b->busy) return false;
+        for (int64_t id : srv->pending)
+            if (srv->reqs[id].d.block == b) return false;
+        return true;
+    });
     srv->arena.release(b->off, b->bytes);
+    srv->blocks.erase(std::remove(srv->blocks.begin(), srv->blocks.end(), b), srv->blocks.end());
     delete b;
 }
 
