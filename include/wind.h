@@ -110,7 +110,9 @@ enum wind_builtin {
     WIND_NET_KATZ96 = 0,      /* N = 5, 10 constants per system */
     WIND_NET_NR_TEST = 1,     /* N = 2, 4 constants per system */
     WIND_NET_HHE6 = 2,        /* N = 6, 1 constant (n_H); 24 network params */
-    WIND_NET_MASS_ACTION = 3  /* N species, R reactions a+b->c+d; 1 constant */
+    WIND_NET_MASS_ACTION = 3, /* N species, R reactions a+b->c+d; 1 constant */
+    WIND_NET_PLUGIN = 4       /* generated by wind_amd/codegen.py; see
+                                 wind_network_create_plugin */
 };
 
 typedef struct wind_network wind_network;
@@ -120,6 +122,19 @@ typedef struct wind_network wind_network;
  * otherwise. */
 int wind_network_create_builtin(int builtin, const float *params, int nparams,
                                 wind_network **out);
+
+/* Loads a network compiled by the code generator (wind_amd/codegen.py)
+ * from its plugin library (wind_amd/lib/gen/<name>/libwind_net.so): the
+ * network's RHS and Jacobian as straight-line code, integrated by the
+ * lane-per-system kernels like the compiled-in networks.  Replaces the
+ * reference's network precompiler (python/ode_systems/ode_base.py:16-123),
+ * which rewrites the CUDA sources and rebuilds the libraries per network.
+ * WIND_ERR_ARGUMENT if the file cannot be loaded or was built against
+ * another version of this library. */
+int wind_network_create_plugin(const char *path, wind_network **out);
+
+/* Constants per system of a network (its NUM_CONST). */
+int wind_network_nconst(const wind_network *net);
 
 /* Creates a mass-action network: reaction r is rx[4r] + rx[4r+1] ->
  * rx[4r+2] + rx[4r+3] with rate k[r] * c[0] * y_a * y_b.  The library copies

@@ -7,6 +7,8 @@
 // a comment cites the line each block restates.
 #include "wind_oracle.h"
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -332,6 +334,12 @@ void net_rhs(const oracle_net *net, R /*t*/, const R *y, const R *c, R *f) {
     case ORACLE_NET_NR_TEST: nr_test_rhs(y, c, f); break;
     case ORACLE_NET_HHE6: hhe6_rhs(y, c, net->params, f); break;
     case ORACLE_NET_MASS_ACTION: mass_action_rhs(net, y, c, f); break;
+    case ORACLE_NET_GENERATED:
+        if constexpr (std::is_same<R, float>::value)
+            net->gen_rhs_f32(y, c, f);
+        else
+            net->gen_rhs_f64(y, c, f);
+        break;
     default: std::abort();
     }
 }
@@ -343,6 +351,12 @@ void net_jac(const oracle_net *net, const R *y, const R *c, R *J) {
     case ORACLE_NET_NR_TEST: nr_test_jac(y, c, J); break;
     case ORACLE_NET_HHE6: hhe6_jac(y, c, net->params, J); break;
     case ORACLE_NET_MASS_ACTION: mass_action_jac(net, y, c, J); break;
+    case ORACLE_NET_GENERATED:
+        if constexpr (std::is_same<R, float>::value)
+            net->gen_jac_f32(y, c, J);
+        else
+            net->gen_jac_f64(y, c, J);
+        break;
     default: std::abort();
     }
 }

@@ -44,7 +44,8 @@ enum oracle_network {
     ORACLE_NET_KATZ96 = 0,      /* N = 5, 10 constants (ode_gold.c) */
     ORACLE_NET_NR_TEST = 1,     /* N = 2, 4 constants (NR_test.py) */
     ORACLE_NET_HHE6 = 2,        /* N = 6, 1 constant: n_H */
-    ORACLE_NET_MASS_ACTION = 3  /* table driven, 1 constant: rate scale */
+    ORACLE_NET_MASS_ACTION = 3, /* table driven, 1 constant: rate scale */
+    ORACLE_NET_GENERATED = 4    /* functions written by wind_amd/codegen.py */
 };
 
 enum oracle_method { ORACLE_SIE = 0, ORACLE_RK2 = 1 };
@@ -69,6 +70,12 @@ typedef struct oracle_net {
     int R;                    /* mass_action reactions */
     const int32_t *rx;        /* [R*4]: reactant a, reactant b, product c, product d */
     const float *k;           /* [R] rate constants */
+    /* generated networks: RHS and column-major Jacobian (J accumulated onto
+     * zeros) in float and double, from oracle/_build/gen/<name>.so */
+    void (*gen_rhs_f32)(const float *y, const float *c, float *f);
+    void (*gen_jac_f32)(const float *y, const float *c, float *J);
+    void (*gen_rhs_f64)(const double *y, const double *c, double *f);
+    void (*gen_jac_f64)(const double *y, const double *c, double *J);
 } oracle_net;
 
 /* One attempt of the step-doubling loop, recorded for trace fixtures. */

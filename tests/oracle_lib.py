@@ -13,13 +13,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libwind_oracle.so")
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 
-NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION = 0, 1, 2, 3
+NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION, NET_GENERATED = 0, 1, 2, 3, 4
+GEN_DIR = os.path.join(ROOT, "oracle", "_build", "gen")
 SIE, RK2 = 0, 1
 
 
 class OracleNet(C.Structure):
     _fields_ = [("id", C.c_int), ("N", C.c_int), ("params", C.c_void_p), ("R", C.c_int),
-                ("rx", C.c_void_p), ("k", C.c_void_p)]
+                ("rx", C.c_void_p), ("k", C.c_void_p), ("gen_rhs_f32", C.c_void_p), ("gen_jac_f32", C.c_void_p),
+                ("gen_rhs_f64", C.c_void_p), ("gen_jac_f64", C.c_void_p)]
 
 
 class OracleTrace(C.Structure):
@@ -98,6 +100,18 @@ def hhe6(params):
 
 def mass_action(N, rx, k):
     return Net(NET_MASS_ACTION, N, rx=rx, k=k)
+
+
+def generated(name):
+    """A network written by wind_amd/codegen.py (its oracle statement,
+    oracle/_build/gen/<name>.so)."""
+    g = C.CDLL(os.path.join(GEN_DIR, f"{name}.so"))
+    net = Net(NET_GENERATED, g.gen_N())
+    net.gen = g  # keep the library loaded
+    net.nconst = g.gen_nconst()
+    for fld in ("gen_rhs_f32", "gen_jac_f32", "gen_rhs_f64", "gen_jac_f64"):
+        setattr(net.s, fld, C.cast(getattr(g, fld), C.c_void_p).value)
+    return net
 
 
 def _f32(a):

@@ -1,0 +1,161 @@
+"""Network code generator (wind_amd/codegen.py), without a GPU.
+
+* The generated oracle statement of a two-body network computes the same
+  bits as the oracle's table-driven mass-action network (float and double):
+  the generator writes the table path's operation order.
+* A general network (one- and two-body reactions, two constants) against an
+  independent float64 evaluation of its description, and its Jacobian
+  against finite differences of its RHS.
+* Operation counts, the plugin descriptor, the per-network shims' exported
+  names, and rejection of malformed descriptions.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from wind_amd import codegen
+from wind_amd import networks as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GEN = os.path.join(ROOT, "wind_amd", "lib", "gen")
+
+
+def _eval(net, y, c, dtype):
+    L = O.lib()
+    N = net.N
+    f = np.zeros(N, dtype)
+    J = np.zeros(N * N, dtype)
+    y = np.ascontiguousarray(y, dtype)
+    c = np.ascontiguousarray(c, dtype)
+    if dtype == np.float32:
+        L.oracle_rhs(net.ref, C.c_float(0.0), y.ctypes.data, c.ctypes.data, f.ctypes.data)
+        L.oracle_jac(net.ref, y.ctypes.data, c.ctypes.data, J.ctypes.data)
+    else:
+        L.oracle_rhs_f64(net.ref, C.c_double(0.0), y.ctypes.data, c.ctypes.data, f.ctypes.data)
+        L.oracle_jac_f64(net.ref, y.ctypes.data, c.ctypes.data, J.ctypes.data)
+    return f, J
+
+
+@pytest.mark.parametrize("N", list(W.GENERATED_MIXED))
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_generated_equals_table(N, dtype):
+    rx, k = W.mixed_network(N)
+    gen = O.generated(f"mixed{N}")
+    tab = O.mass_action(N, rx, k)
+    rng = np.random.default_rng(N)
+    for trial in range(20):
+        y = np.exp(rng.uniform(np.log(1e-12), 0.0, N)).astype(np.float32)
+        c = np.array([np.exp(rng.uniform(np.log(0.1), np.log(10.0)))], np.float32)
+        fg, Jg = _eval(gen, y, c, dtype)
+        ft, Jt = _eval(tab, y, c, dtype)
+        assert np.array_equal(fg.view(np.uint8), ft.view(np.uint8)), trial
+        assert np.array_equal(Jg.view(np.uint8), Jt.view(np.uint8)), trial
+
+
+def _reference_eval(spec, y, c):
+    """float64 RHS and Jacobian straight from the description."""
+    N = spec["species"]
+    f = np.zeros(N)
+    J = np.zeros((N, N))  # J[i, j] = d f_i / d y_j
+    for rx in spec["reactions"]:
+        kc = float(np.float32(rx["k"])) * (c[rx["const"]] if rx["const"] is not None else 1.0)
+        reac = rx["reactants"]
+        w = kc * np.prod([y[s] for s in reac])
+        dw = np.zeros(N)
+        if len(reac) == 2:
+            a, b = reac
+            dw[a] += kc * y[b]
+            dw[b] += kc * y[a]
+        else:
+            dw[reac[0]] += kc
+        for s in reac:
+            f[s] -= w
+            J[s] -= dw
+        for s in rx["products"]:
+            f[s] += w
+            J[s] += dw
+    return f, J
+
+
+def test_general_network_against_description():
+    spec = [s for s in W.generated_specs() if s["name"] == "demo_decay"][0]
+    gen = O.generated("demo_decay")
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        y = rng.uniform(0.01, 2.0, 4)
+        c = rng.uniform(0.5, 2.0, 2)
+        f, Jc = _eval(gen, y, c, np.float64)
+        fr, Jr = _reference_eval(spec, y, c)
+        assert np.allclose(f, fr, rtol=1e-13, atol=1e-15)
+        J = Jc.reshape(4, 4).T  # stored column-major
+        assert np.allclose(J, Jr, rtol=1e-13, atol=1e-15)
+        # finite differences of the generated RHS
+        eps = 1e-6
+        for j in range(4):
+            e = np.zeros(4)
+            e[j] = eps
+            fp, _ = _eval(gen, y + e, c, np.float64)
+            fm, _ = _eval(gen, y - e, c, np.float64)
+            assert np.allclose((fp - fm) / (2 * eps), J[:, j], rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["mixed5", "mixed10", "demo_decay"])
+def test_plugin_descriptor_and_counts(name):
+    spec = [s for s in W.generated_specs() if s["name"] == name][0]
+    g = codegen.generate(spec)
+    meta = json.load(open(os.path.join(GEN, name, "network.json")))
+    assert meta["flops_rhs"] == g["flops_rhs"] and meta["flops_jac"] == g["flops_jac"]
+    if name.startswith("mixed"):
+        R = len(spec["reactions"])
+        assert g["flops_rhs"] == 7 * R and g["flops_jac"] == 11 * R
+
+    class Desc(C.Structure):
+        _fields_ = [("abi", C.c_int), ("N", C.c_int), ("nconst", C.c_int), ("nreactions", C.c_int),
+                    ("name", C.c_char_p), ("flops_rhs", C.c_longlong), ("flops_jac", C.c_longlong),
+                    ("launch", C.c_void_p), ("lockstep_attempt", C.c_void_p)]
+
+    lib = C.CDLL(os.path.join(GEN, name, "libwind_net.so"))
+    lib.wind_plugin_get.restype = C.POINTER(Desc)
+    d = lib.wind_plugin_get().contents
+    assert d.abi == 1 and d.N == g["N"] and d.nconst == g["nconst"] and d.nreactions == g["R"]
+    assert d.name.decode() == name and d.flops_rhs == g["flops_rhs"] and d.flops_jac == g["flops_jac"]
+    assert d.launch and d.lockstep_attempt
+
+
+@pytest.mark.parametrize("name", ["mixed5", "demo_decay"])
+def test_generated_shims_export_reference_symbols(name):
+    import subprocess
+
+    def syms(path):
+        out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+        return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+    d = os.path.join(GEN, name)
+    assert "_Z19cudaIntegrateSystemffiPfS_iiff" in syms(os.path.join(d, "sie.so"))
+    assert "_Z19cudaIntegrateSystemffiPfS_iiff" in syms(os.path.join(d, "rk2.so"))
+    host = syms(os.path.join(d, "sie_host.so"))
+    assert "_Z16cudaIntegrateSIEffiPfS_iiff" in host and "_Z26initializeCublasExternallyv" in host
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ({"name": "x-y", "species": 2, "reactions": []}, "identifier"),
+    ({"name": "x", "species": 2, "reactions": [{"reactants": [0, 1, 1], "products": [], "k": 1.0}]}, "reactants"),
+    ({"name": "x", "species": 2, "reactions": [{"reactants": [2], "products": [], "k": 1.0}]}, "out of range"),
+    ({"name": "x", "species": 2, "nconst": 1, "reactions": [{"reactants": [0], "products": [1], "k": 1.0,
+                                                             "const": 1}]}, "constant"),
+    ({"name": "x", "species": 2, "reactions": [{"reactants": [0], "products": [1], "k": float("inf")}]},
+     "non-finite"),
+])
+def test_rejects_malformed(bad, msg):
+    with pytest.raises(ValueError, match=msg):
+        codegen.generate(bad)
+
+
+def test_float_literals_round_trip():
+    for v in (1.0, 3.0e-2, 1e-18, 7.5e-7, 123456.789, 2.0**-126):
+        lit = codegen._lit(v)
+        assert np.float32(float(lit[:-1])) == np.float32(v), lit

@@ -26,7 +26,8 @@ STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 
 EXPORTED = (
     "wind_network_create_builtin", "wind_network_create_mass_action", "wind_network_destroy",
-    "wind_network_size", "wind_options_init", "wind_integrate", "wind_integrate_lockstep",
+    "wind_network_size", "wind_network_nconst", "wind_network_create_plugin", "wind_options_init",
+    "wind_integrate", "wind_integrate_lockstep",
     "wind_last_error", "wind_version", "wind_integrate_batch", "wind_server_create", "wind_server_destroy",
     "wind_server_block_create", "wind_server_block_read", "wind_server_block_destroy", "wind_server_submit",
     "wind_server_wait", "wind_server_inflight",
@@ -83,6 +84,8 @@ def load(path: str = LIB_PATH):
     lib.wind_network_destroy.argtypes = [C.c_void_p]
     lib.wind_network_destroy.restype = None
     lib.wind_network_size.argtypes = [C.c_void_p]
+    lib.wind_network_nconst.argtypes = [C.c_void_p]
+    lib.wind_network_create_plugin.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
     lib.wind_options_init.argtypes = [C.POINTER(WindOptions)]
     lib.wind_options_init.restype = None
     lib.wind_integrate.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,
@@ -155,6 +158,18 @@ class Network:
         h = C.c_void_p()
         _check(lib.wind_network_create_mass_action(N, len(k), rx.ctypes.data, k.ctypes.data, C.byref(h)))
         return cls(h, N, 1, f"mass_action_{N}")
+
+    @classmethod
+    def plugin(cls, name_or_path):
+        """A network generated by wind_amd/codegen.py: its name (looked up in
+        wind_amd/lib/gen/<name>/) or the path of its libwind_net.so."""
+        lib = load()
+        path = name_or_path
+        if not path.endswith(".so"):
+            path = os.path.join(_HERE, "lib", "gen", name_or_path, "libwind_net.so")
+        h = C.c_void_p()
+        _check(lib.wind_network_create_plugin(path.encode(), C.byref(h)))
+        return cls(h, lib.wind_network_size(h), lib.wind_network_nconst(h), os.path.basename(os.path.dirname(path)))
 
     def close(self):
         if self.handle:

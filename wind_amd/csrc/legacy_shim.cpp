@@ -6,13 +6,17 @@
 //
 // Build-time definitions:
 //   WIND_SHIM_KIND       1 = sie.so, 2 = rk2.so, 3 = sie_host.so
-//   WIND_SHIM_NETWORK    a wind_builtin value
+//   WIND_SHIM_NETWORK    a wind_builtin value; 4 = a generated network whose
+//                        plugin (libwind_net.so) sits beside this library
 //   WIND_SHIM_NUM_CONST  constant stride per system (the reference's NUM_CONST)
 //   WIND_SHIM_PARAMS     header defining wind_shim_params[] (hhe6) or the
 //                        reaction table wind_shim_rx[] / wind_shim_k[] and
 //                        WIND_SHIM_N / WIND_SHIM_R (mass action)
+#include <dlfcn.h>
+
 #include <cstdio>
 #include <mutex>
+#include <string>
 
 #include "../../include/wind.h"
 
@@ -27,7 +31,16 @@ wind_network *shim_network() {
     static wind_network *net = nullptr;
     std::call_once(once, [] {
         int rc;
-#if WIND_SHIM_NETWORK == 3
+#if WIND_SHIM_NETWORK == 4
+        std::string path = "libwind_net.so";
+        Dl_info info;
+        if (dladdr((void *)&shim_network, &info) && info.dli_fname) {
+            const std::string self(info.dli_fname);
+            const size_t slash = self.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : self.substr(0, slash)) + "/libwind_net.so";
+        }
+        rc = wind_network_create_plugin(path.c_str(), &net);
+#elif WIND_SHIM_NETWORK == 3
         rc = wind_network_create_mass_action(WIND_SHIM_N, WIND_SHIM_R, wind_shim_rx, wind_shim_k, &net);
 #elif WIND_SHIM_NETWORK == 2
         rc = wind_network_create_builtin(WIND_SHIM_NETWORK, wind_shim_params, 24, &net);

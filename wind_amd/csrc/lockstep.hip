@@ -6,13 +6,10 @@
 // returns with the systems at the last accepted time.  The return value is
 // 3 * attempts * Nsystems (harness_kernel.cu:295,424).
 //
-// Each attempt is one kernel launch over all systems (one system per lane,
-// the same register-resident sub-step as the per-system integrator) that
-// writes the two-half-step result and raises a device flag when a system
-// fails; the host reads the 4-byte flag and either commits the half-step
-// state or halves the step.  The reference's error test here differs from
-// c_baseline's: strict '>' comparisons, and the relative test requires
-// y1 > ABS and y2 > ABS without absolute values (vector_kernels.cu:45-70).
+// Each attempt is one kernel launch over all systems (kernels_lockstep.h)
+// that writes the two-half-step result and raises a device flag when a
+// system fails; the host reads the 4-byte flag and either commits the
+// half-step state or halves the step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,44 +18,13 @@
 #include <mutex>
 #include <string>
 
-#include "kernels_small.h"
+#include "kernels_lockstep.h"
+#include "plugin_abi.h"
 #include "wind_internal.h"
 
 using namespace wind;
 
 namespace {
-
-template <class Net>
-__global__ void __launch_bounds__(256)
-    lockstep_attempt(float tnow, float hh, const float *__restrict__ C, int stride,
-                     const float *__restrict__ Y, float *__restrict__ Y2, int nsys, float ABS,
-                     float REL, int *flag, typename Net::Params p) {
-    constexpr int N = Net::N;
-    constexpr int NC = Net::NCONST;
-    const float tb = tnow + hh;
-    const float hs1 = tb - tnow;          // solveSystem(tnow, tnow + h, 1)
-    const float hs2 = (tb - tnow) / 2.0f; // solveSystem(tnow, tnow + h, 2)
-    bool fail = false;
-    for (int sid = blockIdx.x * blockDim.x + threadIdx.x; sid < nsys; sid += gridDim.x * blockDim.x) {
-        float y1[N], y2[N], c[NC];
-#pragma unroll
-        for (int i = 0; i < N; i++) y1[i] = y2[i] = Y[(size_t)sid * N + i];
-#pragma unroll
-        for (int k = 0; k < NC; k++) c[k] = C[(size_t)sid * stride + k];
-        substep<Net, WIND_SIE, MathExact>(p, c, y1, hs1);
-        substep<Net, WIND_SIE, MathExact>(p, c, y2, hs2);
-        substep<Net, WIND_SIE, MathExact>(p, c, y2, hs2);
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            const float d = y1[i] - y2[i];
-            if (fabsf(d) > ABS) fail = true;
-            if (fabs((double)d / ((double)y2[i] + 1e-12)) > (double)REL && y1[i] > ABS && y2[i] > ABS)
-                fail = true;
-            Y2[(size_t)sid * N + i] = y2[i];
-        }
-    }
-    if (__any(fail) && __lane_id() == 0) atomicOr(flag, 1);
-}
 
 struct LockstepBuffers {
     std::mutex mu;
@@ -84,14 +50,32 @@ int grow(float **p, size_t *have, size_t need) {
     return WIND_OK;
 }
 
+// One attempt over all systems: launches the network's lockstep kernel.
+using AttemptFn = int (*)(const wind_network *net, float tnow, float hh, const float *C, int stride,
+                          const float *Y, float *Y2, int nsys, float ABS, float REL, int *flag, int grid);
+
 template <class Net>
-int run(const wind_network *net, float tnow, float tend, int nfix, const float *constants,
-        float *equations, int ns, float ABS, float REL, int stride, int64_t *result) {
-    constexpr int N = Net::N;
+int attempt_builtin(const wind_network *net, float tnow, float hh, const float *C, int stride, const float *Y,
+                    float *Y2, int nsys, float ABS, float REL, int *flag, int grid) {
     typename Net::Params p{};
     if constexpr (sizeof(typename Net::Params) > 1) std::memcpy(&p, net->params.data(), sizeof(p));
+    hipLaunchKernelGGL(lockstep_attempt<Net>, dim3(grid), dim3(256), 0, 0, tnow, hh, C, stride, Y, Y2, nsys, ABS,
+                       REL, flag, p);
+    LS_HIP(hipGetLastError());
+    return WIND_OK;
+}
+
+int attempt_plugin(const wind_network *net, float tnow, float hh, const float *C, int stride, const float *Y,
+                   float *Y2, int nsys, float ABS, float REL, int *flag, int grid) {
+    const int rc = net->plugin->lockstep_attempt(tnow, hh, C, stride, Y, Y2, nsys, ABS, REL, flag, grid);
+    return rc ? wind_fail(rc, "generated network: lockstep launch failed") : WIND_OK;
+}
+
+int run(const wind_network *net, AttemptFn attempt, float tnow, float tend, int nfix, const float *constants,
+        float *equations, int ns, float ABS, float REL, int stride, int64_t *result) {
+    const int N = net->N;
     const size_t yb = (size_t)ns * N * sizeof(float);
-    const size_t cb = ((size_t)(ns - 1) * stride + Net::NCONST) * sizeof(float);
+    const size_t cb = ((size_t)(ns - 1) * stride + net->nconst) * sizeof(float);
     int rc = grow(&g_ls.C, &g_ls.cb, cb);
     if (!rc) rc = grow(&g_ls.Y, &g_ls.yb, yb);
     if (!rc) rc = grow(&g_ls.Y2, &g_ls.y2b, yb);
@@ -110,9 +94,8 @@ int run(const wind_network *net, float tnow, float tend, int nfix, const float *
         timestep = fminf(timestep, tend - tnow);
         nsteps += 3;
         LS_HIP(hipMemset(g_ls.flag, 0, sizeof(int)));
-        hipLaunchKernelGGL(lockstep_attempt<Net>, dim3(grid), dim3(256), 0, 0, tnow, timestep, g_ls.C,
-                           stride, g_ls.Y, g_ls.Y2, ns, ABS, REL, g_ls.flag, p);
-        LS_HIP(hipGetLastError());
+        rc = attempt(net, tnow, timestep, g_ls.C, stride, g_ls.Y, g_ls.Y2, ns, ABS, REL, g_ls.flag, grid);
+        if (rc) return rc;
         int flag = 0;
         LS_HIP(hipMemcpy(&flag, g_ls.flag, sizeof(int), hipMemcpyDeviceToHost));
         if (flag) {
@@ -144,17 +127,14 @@ extern "C" int wind_integrate_lockstep(const wind_network *net, float tnow, floa
     if (!constants || !equations) return wind_fail(WIND_ERR_ARGUMENT, "NULL state or constants");
     const int stride = const_stride > 0 ? const_stride : net->nconst;
     std::lock_guard<std::mutex> lock(g_ls.mu);
+    AttemptFn fn = nullptr;
     switch (net->builtin) {
-    case WIND_NET_NR_TEST:
-        return run<NetNRTest>(net, tnow, tend, n_integration_steps, constants, equations, Nsystems,
-                              ABSOLUTE, RELATIVE, stride, result);
-    case WIND_NET_KATZ96:
-        return run<NetKatz96>(net, tnow, tend, n_integration_steps, constants, equations, Nsystems,
-                              ABSOLUTE, RELATIVE, stride, result);
-    case WIND_NET_HHE6:
-        return run<NetHHe6>(net, tnow, tend, n_integration_steps, constants, equations, Nsystems,
-                            ABSOLUTE, RELATIVE, stride, result);
-    default:
-        return wind_fail(WIND_ERR_UNSUPPORTED, "lockstep SIE is built for the compiled networks");
+    case WIND_NET_NR_TEST: fn = attempt_builtin<NetNRTest>; break;
+    case WIND_NET_KATZ96: fn = attempt_builtin<NetKatz96>; break;
+    case WIND_NET_HHE6: fn = attempt_builtin<NetHHe6>; break;
+    case WIND_NET_PLUGIN: fn = attempt_plugin; break;
+    default: return wind_fail(WIND_ERR_UNSUPPORTED, "lockstep SIE is built for the compiled and generated networks");
     }
+    return run(net, fn, tnow, tend, n_integration_steps, constants, equations, Nsystems, ABSOLUTE, RELATIVE, stride,
+               result);
 }

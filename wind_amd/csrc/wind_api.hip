@@ -6,6 +6,7 @@
 // back and frees on every call.  Here device buffers are pooled per process
 // (grow-only, guarded by a mutex) and reused across calls, and callers that
 // keep their state on the device pass device pointers and skip the copies.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +20,7 @@
 #include <vector>
 
 #include "kernels_small.h"
+#include "plugin_abi.h"
 #include "wind_internal.h"
 
 using namespace wind;
@@ -137,6 +139,11 @@ int dispatch(const Context &c, const wind_network *net, const IntegrateArgs &a, 
     case WIND_NET_KATZ96: return launch_small<NetKatz96, METHOD>(c, net, a, mode, kernel, sc, st);
     case WIND_NET_HHE6: return launch_small<NetHHe6, METHOD>(c, net, a, mode, kernel, sc, st);
     case WIND_NET_MASS_ACTION: return launch_mass_action<METHOD>(c.num_cu, net, a, mode, kernel, sc, st);
+    case WIND_NET_PLUGIN: {
+        const int rc = net->plugin->launch(&a, METHOD, mode, kernel, c.num_cu, sc.share, sc.grid, st);
+        if (rc == WIND_ERR_UNSUPPORTED) return fail(rc, "generated networks: the block kernel is exact mode only");
+        return rc ? fail(rc, "generated network: kernel launch failed") : WIND_OK;
+    }
     default: return fail(WIND_ERR_UNSUPPORTED, "unknown network");
     }
 }
@@ -222,13 +229,42 @@ int wind_network_create_mass_action(int N, int R, const int32_t *rx, const float
     return WIND_OK;
 }
 
+int wind_network_create_plugin(const char *path, wind_network **out) {
+    if (!out) return fail(WIND_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!path) return fail(WIND_ERR_ARGUMENT, "path is NULL");
+    Context &c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    int rc = ensure_ready(c);
+    if (rc) return rc;
+    void *dl = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!dl) return fail(WIND_ERR_ARGUMENT, (std::string("cannot load network plugin: ") + dlerror()).c_str());
+    auto get = (wind_plugin_get_fn)dlsym(dl, "wind_plugin_get");
+    const wind_plugin_desc *d = get ? get() : nullptr;
+    if (!d || d->abi != WIND_PLUGIN_ABI || d->N < 1 || d->nconst < 0 || !d->launch || !d->lockstep_attempt) {
+        dlclose(dl);
+        return fail(WIND_ERR_ARGUMENT, "not a network plugin of this library version");
+    }
+    wind_network *n = new wind_network();
+    n->builtin = WIND_NET_PLUGIN;
+    n->N = d->N;
+    n->nconst = d->nconst;
+    n->dl = dl;
+    n->plugin = d;
+    *out = n;
+    return WIND_OK;
+}
+
 void wind_network_destroy(wind_network *net) {
     if (!net) return;
     mass_action_release(net);
+    if (net->dl) dlclose(net->dl);
     delete net;
 }
 
 int wind_network_size(const wind_network *net) { return net ? net->N : 0; }
+
+int wind_network_nconst(const wind_network *net) { return net ? net->nconst : 0; }
 
 int wind_integrate(const wind_network *net, float tnow, float tend, int n_integration_steps,
                    const float *constants, float *equations, int Nsystems, int Neqn,

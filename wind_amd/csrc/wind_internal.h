@@ -27,6 +27,8 @@ struct MassActionDevice {
 
 }  // namespace wind
 
+struct wind_plugin_desc;
+
 struct wind_network {
     int builtin = 0;
     int N = 0;
@@ -36,6 +38,9 @@ struct wind_network {
     // Host copies of the mass-action table (kept for introspection).
     std::vector<int32_t> rx;
     std::vector<float> k;
+    // Generated networks: the plugin library and its descriptor.
+    void *dl = nullptr;
+    const wind_plugin_desc *plugin = nullptr;
 };
 
 int wind_fail(int code, const char *msg);

@@ -123,7 +123,7 @@ def mixed_workload(nsys: int, seed: int = 4):
     out = []
     for N in MIXED_SIZES:
         idx = np.nonzero(Ns == N)[0]
-        rx, k = mass_action_network(N, seed=1000 + N)
+        rx, k = mixed_network(N)
         y0, sc = mass_action_systems(len(idx), N, seed=2000 + N + seed, scale_range=MIXED_SCALE_RANGE)
         out.append({"N": N, "rx": rx, "k": k, "y0": y0, "scale": sc, "index": idx})
     return out
@@ -142,4 +142,37 @@ def dense_workload(nsys: int = DENSE_NSYS):
     rx, k = mass_action_network(DENSE_N, seed=DENSE_NET_SEED)
     y0, scale = mass_action_systems(nsys, DENSE_N, seed=DENSE_SYS_SEED)
     return rx, k, y0, scale
+
+
+# ------------------------------------------------------- generated networks
+GENERATED_MIXED = (5, 10)  # mixed-workload networks also built as generated plugins
+
+
+def mixed_network(N: int):
+    """The mixed workload's network for size N (rx, k)."""
+    return mass_action_network(N, seed=1000 + N)
+
+
+def generated_specs():
+    """Descriptions of the networks that the build generates: the mixed
+    workload's N = 5 and N = 10 networks (as "mixed5", "mixed10"), and a small
+    network with one- and two-body reactions and two constants per system
+    ("demo_decay")."""
+    from wind_amd import codegen
+
+    specs = []
+    for N in GENERATED_MIXED:
+        rx, k = mixed_network(N)
+        specs.append(codegen.mass_action_spec(f"mixed{N}", N, rx, k))
+    specs.append({
+        "name": "demo_decay", "species": 4, "nconst": 2,
+        "reactions": [
+            {"reactants": [0], "products": [1], "k": 1.0, "const": 0},
+            {"reactants": [1], "products": [2], "k": 3.0e-2, "const": None},
+            {"reactants": [1, 1], "products": [3], "k": 0.5, "const": 1},
+            {"reactants": [2, 3], "products": [0, 1], "k": 2.0e-3, "const": 1},
+            {"reactants": [3], "products": [], "k": 1.0e-4, "const": 0},
+        ],
+    })
+    return specs
 
