@@ -120,7 +120,7 @@ def ma_rel_err(y, yt, floor=1e-6):
     return np.where(np.isnan(e), np.inf, e).max(axis=1)
 
 
-def mixed_batch(torch, api, W, nsys, steps, plain):
+def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
     """Mixed-N mass-action workload in one batched, autotuned, graph-captured
     call; its accuracy on the FP64-truth sample; the plain one-workgroup-per-
     system kernel (exact arithmetic, same controller and tolerances) timed
@@ -131,7 +131,12 @@ def mixed_batch(torch, api, W, nsys, steps, plain):
     for b in wl:
         n = len(b["scale"])
         y0 = torch.from_numpy(b["y0"]).to(dev)
-        bk.append(dict(net=api.Network.mass_action(b["N"], b["rx"], b["k"]), N=b["N"], nsys=n, y0=y0,
+        table = api.Network.mass_action(b["N"], b["rx"], b["k"])
+        net, kind = table, "table (dense LDS solver)"
+        plugin = os.path.join(ROOT, "wind_amd", "lib", "gen", f"mixed{b['N']}", "libwind_net.so")
+        if use_generated and b["N"] in W.GENERATED_MIXED and os.path.exists(plugin):
+            net, kind = api.Network.plugin(plugin), "generated (lane per system)"
+        bk.append(dict(net=net, table=table, kind=kind, N=b["N"], nsys=n, y0=y0,
                        equations=y0.clone(), constants=torch.from_numpy(b["scale"]).to(dev),
                        steps_out=torch.zeros(n, dtype=torch.int32, device=dev),
                        status_out=torch.zeros(n, dtype=torch.int32, device=dev)))
@@ -161,7 +166,8 @@ def mixed_batch(torch, api, W, nsys, steps, plain):
     for b in bk:
         st = b["status_out"].cpu().numpy()
         sp = b["steps_out"].cpu().numpy().astype(np.int64)
-        info = {"nsys": b["nsys"], "substeps_mean": float(sp.mean()), "substeps_p90": float(np.quantile(sp, 0.9)),
+        info = {"nsys": b["nsys"], "network": b["kind"], "substeps_mean": float(sp.mean()),
+                "substeps_p90": float(np.quantile(sp, 0.9)),
                 "status_ok": int(np.sum(st == 0))}
         if T is not None and nsys == int(T["mixed_nsys"]):
             sel = T[f"mixed_{b['N']}_idx"]
@@ -184,7 +190,7 @@ def mixed_batch(torch, api, W, nsys, steps, plain):
             y = b["y0"][: n * b["N"]].clone()
             torch.cuda.synchronize()
             t = time.perf_counter()
-            api.integrate(b["net"], 0.0, tend, 1, b["constants"][:n], y, n, MA_ABS, MA_REL, mode=api.MODE_EXACT,
+            api.integrate(b["table"], 0.0, tend, 1, b["constants"][:n], y, n, MA_ABS, MA_REL, mode=api.MODE_EXACT,
                           kernel=api.KERNEL_BLOCK, device_pointers=True, controller=api.CTRL_ADAPTIVE,
                           max_attempts=100000)
             torch.cuda.synchronize()
@@ -231,7 +237,11 @@ def serving(api, W, seconds, load_frac, seed=5):
     hp = W.hhe6_params()
     nets = {"hhe6": api.Network.builtin(api.NET_HHE6, hp)}
     for b in W.mixed_workload(1 << 12, seed=4):
-        nets[b["N"]] = (api.Network.mass_action(b["N"], b["rx"], b["k"]), b)
+        plugin = os.path.join(ROOT, "wind_amd", "lib", "gen", f"mixed{b['N']}", "libwind_net.so")
+        if b["N"] in W.GENERATED_MIXED and os.path.exists(plugin):
+            nets[b["N"]] = (api.Network.plugin(plugin), b)
+        else:
+            nets[b["N"]] = (api.Network.mass_action(b["N"], b["rx"], b["k"]), b)
     srv = api.Server(8 << 30, nslots=8)
     blocks = []
     for i in range(24):
@@ -310,6 +320,8 @@ def main():
     ap.add_argument("--mixed-nsys", type=int, default=MIXED_NSYS)
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-generated", action="store_true",
+                    help="mixed batch: table-driven networks for every N (no generated plugins)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--no-serve", action="store_true")
@@ -437,7 +449,8 @@ def main():
                            "launch shape), same controller and tolerance as the headline",
                  "speedup": value / pv}
 
-    mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain)
+    mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain,
+                                                not a.no_generated)
     dense = None if a.no_mixed else dense_roofline(torch, api, W, a.dense_nsys)
     serve = None if a.no_serve else serving(api, W, a.serve_seconds, a.serve_load)
     cpu = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, REF_TOL, 0)
