@@ -174,6 +174,29 @@ __device__ inline void trailing_tiles(float *A, int S, int k0, int kend, int nt,
     }
 }
 
+// c_k = G_k y_k for the 16-row block at k0 (G_k the inverted diagonal
+// block), in place in y; executed by one whole wave (lanes with the same
+// row compute the same value).  The wave reads the block before writing it.
+__device__ inline void block_rhs_apply_g(const float *A, int S, int k0, float *y, int lane) {
+    const int m = lane & 15;
+    float v = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; j++) v = fmaf(A[(k0 + m) * S + k0 + j], y[k0 + j], v);
+    if (lane < 16) y[k0 + m] = v;
+}
+
+// y_i -= A21_ik c_k for the rows i in [lo_row, NP), rows spread over the
+// threads [t0, t0 + nt) of the workgroup.
+__device__ inline void block_rhs_update(const float *A, int S, int k0, int lo_row, int NP, float *y, int t0,
+                                        int nthreads) {
+    for (int i = lo_row + (int)threadIdx.x - t0; i < NP; i += nthreads) {
+        float acc = y[i];
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc = fmaf(-A[i * S + k0 + j], y[k0 + j], acc);
+        y[i] = acc;
+    }
+}
+
 // In-place blocked LU without pivoting of the NP x NP matrix at A (row
 // stride S), called by the whole workgroup, in the block form
 //   A = [I 0; A21 G I] [A11 A12; 0 A22 - A21 G A12],  G = A11^-1.
@@ -182,15 +205,19 @@ __device__ inline void trailing_tiles(float *A, int S, int k0, int kend, int nt,
 // by the earlier panels).  Per panel: W = G A12 in 16x16 tile products,
 // then the trailing update A22 -= A21 W; wave 0 updates the next diagonal
 // block first and inverts it while the other waves finish the update.
+// With a right-hand side y (LDS, NP values) the forward sweep of the solve
+// rides along: per panel c_k = G_k y_k replaces y_k and the rows below are
+// updated, y_i -= A21_ik c_k; dense_back_solve then finishes the solve.
 template <int PREC>
-__device__ inline void dense_lu(float *A, int NP, int S) {
+__device__ inline void dense_lu(float *A, int NP, int S, float *y = nullptr) {
     const int t = threadIdx.x, T = blockDim.x;
     const int wave = t >> 6, lane = t & 63, nw = T >> 6;
     const int lr = lane & 15, g = lane >> 4;
     constexpr int PP = PREC == PREC_VALU ? PREC_VALU : PREC_MFMA_F32;  // W products
     if (wave == 0) diag_block_invert(A, S, 0, lane);
     __syncthreads();
-    for (int k0 = 0; k0 + 16 < NP; k0 += 16) {
+    int k0 = 0;
+    for (; k0 + 16 < NP; k0 += 16) {
         const int kend = k0 + 16;
         const int nt = (NP - kend) >> 4;
         for (int u = wave; u < nt; u += nw) {
@@ -202,6 +229,7 @@ __device__ inline void dense_lu(float *A, int NP, int S) {
 #pragma unroll
             for (int q = 0; q < 4; q++) A[(k0 + 4 * g + q) * S + c0 + lr] = c[q];
         }
+        if (y && wave == nt % nw) block_rhs_apply_g(A, S, k0, y, lane);
         __syncthreads();
         // Tiles 1.. are split over the waves; wave 0 first updates tile 0 (the
         // next diagonal block) and inverts it, so it takes fewer of the rest.
@@ -214,17 +242,66 @@ __device__ inline void dense_lu(float *A, int NP, int S) {
             diag_block_invert(A, S, kend, lane);
         }
         trailing_tiles<PREC>(A, S, k0, kend, nt, lo, hi, lane);
+        if (y) {
+            if (nw == 1)
+                block_rhs_update(A, S, k0, kend, NP, y, 0, 64);
+            else if (wave > 0)
+                block_rhs_update(A, S, k0, kend, NP, y, 64, T - 64);
+        }
+        __syncthreads();
+    }
+    if (y) {
+        if (wave == 0) block_rhs_apply_g(A, S, k0, y, lane);
         __syncthreads();
     }
 }
 
-// Solves A x = b with the block factors left by dense_lu, in one wave
-// (lane l keeps rows l, l + 64, l + 128 in registers; NP <= 192).
-// Forward sweep per 16-row block k: c_k = G_k y_k (block values gathered by
-// readlane), written back to their lanes and broadcast, then
-// y_i -= A21_ik c_k for the rows below.  Backward sweep: x_k = c_k once the
-// blocks after k are done, then c_i -= W_ik x_k for the rows above.
-// b and x may alias; both in LDS.
+// Backward sweep of the block solve in one wave (lane l keeps rows l,
+// l + 64, l + 128 in registers; NP <= 192): x_k = c_k once the blocks after
+// k are done, then c_i -= W_ik x_k for the rows above.  c and x may alias.
+__device__ inline void dense_back_solve(const float *A, int NP, int S, const float *cv, float *x) {
+    const int t = threadIdx.x;
+    if (t < 64) {
+        const int NB = NP >> 4;
+        float xr[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int row = 64 * c + t;
+            xr[c] = row < NP ? cv[row] : 0.0f;
+        }
+#pragma unroll
+        for (int kb = 11; kb > 0; kb--) {
+            if (kb >= NB) continue;
+            const int c = kb >> 2, kq = kb & 3, k0 = kb << 4;
+            float xk[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) xk[j] = bcast(xr[c], 16 * kq + j);
+#pragma unroll
+            for (int c2 = 0; c2 < 3; c2++) {
+                const int row = 64 * c2 + t;
+                if (c2 <= c && row < k0) {
+                    float acc = xr[c2];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) acc = fmaf(-A[row * S + k0 + j], xk[j], acc);
+                    xr[c2] = acc;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int row = 64 * c + t;
+            if (row < NP) x[row] = xr[c];
+        }
+    }
+    __syncthreads();
+}
+
+// Solves A x = b with the block factors left by dense_lu (used by the
+// refinement sweeps; the first solve folds its forward sweep into dense_lu).
+// Forward sweep in one wave, per 16-row block k: c_k = G_k y_k (block values
+// gathered by readlane), written back to their lanes and broadcast, then
+// y_i -= A21_ik c_k for the rows below; then dense_back_solve.  b and x may
+// alias; both in LDS.
 __device__ inline void dense_lu_solve(const float *A, int NP, int S, const float *b, float *x) {
     const int t = threadIdx.x;
     if (t < 64) {
@@ -261,34 +338,15 @@ __device__ inline void dense_lu_solve(const float *A, int NP, int S, const float
             }
         }
 #pragma unroll
-        for (int kb = 11; kb > 0; kb--) {
-            if (kb >= NB) continue;
-            const int c = kb >> 2, kq = kb & 3, k0 = kb << 4;
-            float xk[16];
-#pragma unroll
-            for (int j = 0; j < 16; j++) xk[j] = bcast(xr[c], 16 * kq + j);
-#pragma unroll
-            for (int c2 = 0; c2 < 3; c2++) {
-                const int row = 64 * c2 + t;
-                if (c2 <= c && row < k0) {
-                    float acc = xr[c2];
-#pragma unroll
-                    for (int j = 0; j < 16; j++) acc = fmaf(-A[row * S + k0 + j], xk[j], acc);
-                    xr[c2] = acc;
-                }
-            }
-        }
-#pragma unroll
         for (int c = 0; c < 3; c++) {
             const int row = 64 * c + t;
             if (row < NP) x[row] = xr[c];
         }
     }
     __syncthreads();
+    dense_back_solve(A, NP, S, x, x);
 }
 
-// LDS layout of one workgroup: per-system vectors, per-reaction scratch,
-// the network tables (copied once per workgroup) and the matrix.
 struct DenseLayout {
     float *y, *yw, *y1, *f, *x, *r, *A, *w, *hda, *hdb, *k;
     int4 *rx;
@@ -448,8 +506,12 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
                 }
                 dense_eval(net, L.yw, scale, L, hs, METHOD == WIND_SIE);
                 if (METHOD == WIND_SIE) {
-                    dense_lu<PREC>(L.A, NP, L.S);
-                    dense_lu_solve(L.A, NP, L.S, L.f, L.x);
+                    // x = (I - hJ)^{-1} f: forward sweep inside the factorisation
+                    // (f is overwritten by c = L^{-1} f), then the backward sweep.
+                    for (int i = t; i < NP; i += T) L.r[i] = L.f[i];
+                    __syncthreads();
+                    dense_lu<PREC>(L.A, NP, L.S, L.r);
+                    dense_back_solve(L.A, NP, L.S, L.r, L.x);
                     for (int s = 0; s < ir_sweeps; s++) dense_refine(net, L.yw, scale, L, hs);
                     for (int i = t; i < N; i += T) L.yw[i] = fmaf(hs, L.x[i], L.yw[i]);
                 } else {
