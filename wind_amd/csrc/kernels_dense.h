@@ -91,10 +91,29 @@ __device__ inline float lane_read(float v, int src_lane) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
 }
 
+// Lane n of each 16-lane row, broadcast to the whole row (DPP row_newbcast;
+// k is a compile-time value once the callers' loops are unrolled).
+__device__ inline float row_bcast(float v, int k) {
+    const int x = __float_as_int(v);
+    int r = x;
+    switch (k) {
+#define WIND_ROW_BCAST(n) \
+    case n: r = __builtin_amdgcn_update_dpp(0, x, 0x150 + n, 0xF, 0xF, false); break;
+        WIND_ROW_BCAST(0) WIND_ROW_BCAST(1) WIND_ROW_BCAST(2) WIND_ROW_BCAST(3)
+        WIND_ROW_BCAST(4) WIND_ROW_BCAST(5) WIND_ROW_BCAST(6) WIND_ROW_BCAST(7)
+        WIND_ROW_BCAST(8) WIND_ROW_BCAST(9) WIND_ROW_BCAST(10) WIND_ROW_BCAST(11)
+        WIND_ROW_BCAST(12) WIND_ROW_BCAST(13) WIND_ROW_BCAST(14) WIND_ROW_BCAST(15)
+#undef WIND_ROW_BCAST
+    default: break;
+    }
+    return __int_as_float(r);
+}
+
 // Replaces the 16x16 diagonal block at (k0, k0) by its inverse (in-place
 // Gauss-Jordan without pivoting).  Executed by one whole wave: lane l holds
-// row l % 16, columns 4(l / 16) .. 4(l / 16) + 3; the pivot row and the
-// pivot column are fetched across lanes with ds_bpermute.
+// row l % 16, columns 4(l / 16) .. 4(l / 16) + 3.  The pivot row lies in
+// each lane's own 16-lane row (DPP row broadcast); the pivot column comes
+// from another row of lanes (ds_bpermute).
 __device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
     const int r = lane & 15, g = lane >> 4;
     float a[4];
@@ -105,7 +124,7 @@ __device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
         const int kg = k >> 2, kc = k & 3;
         float pv[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) pv[c] = lane_read(a[c], k + 16 * g);
+        for (int c = 0; c < 4; c++) pv[c] = row_bcast(a[c], k);
         const float f = lane_read(a[kc], r + 16 * kg);
         const float rp = __builtin_amdgcn_rcpf(bcast(a[kc], k + 16 * kg));
 #pragma unroll
@@ -124,7 +143,7 @@ __device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
 
 // Cost of one diagonal-block inversion in trailing-tile updates (balances
 // wave 0's share of the trailing update).
-constexpr int kDiagTiles = 4;
+constexpr int kDiagTiles = 2;
 
 // Trailing update C -= A21 W for tiles [lo, hi) (row-major over nt x nt
 // tiles of the trailing matrix at (kend, kend)); with FP32 MFMA two tiles go
