@@ -146,39 +146,57 @@ __device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
 constexpr int kDiagTiles = 2;
 
 // Trailing update C -= A21 W for tiles [lo, hi) (row-major over nt x nt
-// tiles of the trailing matrix at (kend, kend)); with FP32 MFMA two tiles go
-// at a time so that their load latencies and MFMA chains overlap.
+// tiles of the trailing matrix at (kend, kend)).  With FP32 MFMA the tiles
+// are walked row by row: the A21 operand of a tile row is loaded once, and
+// four tiles go at a time so that their loads and MFMA chains overlap.
 template <int PREC>
 __device__ inline void trailing_tiles(float *A, int S, int k0, int kend, int nt, int lo, int hi, int lane) {
     const int lr = lane & 15, g = lane >> 4;
     int tile = lo;
     if (PREC == PREC_MFMA_F32) {
-        for (; tile + 1 < hi; tile += 2) {
-            const int rb0 = tile / nt, rb1 = (tile + 1) / nt;
-            const int r0 = kend + (rb0 << 4), c0 = kend + ((tile - rb0 * nt) << 4);
-            const int r1 = kend + (rb1 << 4), c1 = kend + ((tile + 1 - rb1 * nt) << 4);
-            f32x4 d0, d1;
-            float a0[4], b0[4], a1[4], b1[4];
+        while (tile < hi) {
+            const int rb = tile / nt;
+            const int seg_end = min(hi, (rb + 1) * nt);
+            const int r0 = kend + (rb << 4);
+            float xa[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                d0[q] = A[(r0 + 4 * g + q) * S + c0 + lr];
-                d1[q] = A[(r1 + 4 * g + q) * S + c1 + lr];
-                a0[q] = -A[(r0 + lr) * S + k0 + 4 * q + g];
-                b0[q] = A[(k0 + 4 * q + g) * S + c0 + lr];
-                a1[q] = -A[(r1 + lr) * S + k0 + 4 * q + g];
-                b1[q] = A[(k0 + 4 * q + g) * S + c1 + lr];
+            for (int q = 0; q < 4; q++) xa[q] = -A[(r0 + lr) * S + k0 + 4 * q + g];
+            for (; tile + 3 < seg_end; tile += 4) {
+                const int c0 = kend + ((tile - rb * nt) << 4);
+                f32x4 d[4];
+                float b[4][4];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        d[u][q] = A[(r0 + 4 * g + q) * S + c0 + 16 * u + lr];
+                        b[u][q] = A[(k0 + 4 * q + g) * S + c0 + 16 * u + lr];
+                    }
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) d[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[q], b[u][q], d[u], 0, 0, 0);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) A[(r0 + 4 * g + q) * S + c0 + 16 * u + lr] = d[u][q];
             }
+            for (; tile < seg_end; tile++) {
+                const int c0 = kend + ((tile - rb * nt) << 4);
+                f32x4 d;
+                float b[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[q], b0[q], d0, 0, 0, 0);
-                d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[q], b1[q], d1, 0, 0, 0);
-            }
+                for (int q = 0; q < 4; q++) {
+                    d[q] = A[(r0 + 4 * g + q) * S + c0 + lr];
+                    b[q] = A[(k0 + 4 * q + g) * S + c0 + lr];
+                }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                A[(r0 + 4 * g + q) * S + c0 + lr] = d0[q];
-                A[(r1 + 4 * g + q) * S + c1 + lr] = d1[q];
+                for (int q = 0; q < 4; q++) d = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[q], b[q], d, 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < 4; q++) A[(r0 + 4 * g + q) * S + c0 + lr] = d[q];
             }
         }
+        return;
     }
     for (; tile < hi; tile++) {
         const int rb = tile / nt;
