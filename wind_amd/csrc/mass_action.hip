@@ -109,6 +109,42 @@ int launch_dense(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, 
     return WIND_OK;
 }
 
+// Which kernel launch_mass_action picks, and whether it needs the
+// process-wide matrix scratch (only callers holding the context lock may
+// use that).
+struct MassActionPlan {
+    bool dense = false;      // ma_dense_integrate
+    bool in_lds = true;      // ma_block_integrate: matrices in LDS
+    size_t lds = 0;
+};
+
+static MassActionPlan mass_action_plan(const MassActionDevice &d, int mode) {
+    MassActionPlan p;
+    int dev = 0, max_lds = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const int N = d.N;
+    const bool fast = mode == WIND_MODE_FAST;
+    if (fast && (N + 15) / 16 * 16 <= 192) {
+        const size_t lds = dense_lds_bytes(N, d.R);
+        if (lds + 128 <= (size_t)max_lds) {
+            p.dense = true;
+            p.lds = lds;
+            return p;
+        }
+    }
+    const size_t base = sizeof(float) * (5 * (size_t)N + 3 * (size_t)d.R);
+    const size_t mats = sizeof(float) * (fast ? 1 : 2) * (size_t)N * N;
+    p.in_lds = base + mats + 64 <= (size_t)max_lds;
+    p.lds = base + (p.in_lds ? mats : 0);
+    return p;
+}
+
+bool mass_action_needs_scratch(const wind_network *net, int mode) {
+    const MassActionPlan p = mass_action_plan(net->ma, mode);
+    return !p.dense && !p.in_lds;
+}
+
 template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
                        int kernel, const SolverChoice &sc, hipStream_t st) {
@@ -117,33 +153,28 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
     const MassActionDevice &d = net->ma;
     const int N = d.N;
     const bool fast = mode == WIND_MODE_FAST;
-    int dev = 0;
-    hipGetDevice(&dev);
-    int max_lds = 0;
-    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    if (fast && (N + 15) / 16 * 16 <= 192) {
-        const size_t lds = dense_lds_bytes(N, d.R);
-        if (lds + 128 <= (size_t)max_lds) {
-            int T = ((N + 15) & ~15) <= 64 ? 64 : 512;
-            if (sc.block == 64 || sc.block == 128 || sc.block == 256 || sc.block == 512) T = sc.block;
-            const SolverChoice &g = sc;
-            int solver = sc.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : sc.solver;
-            switch (solver) {
-            case WIND_SOLVER_VALU: return launch_dense<METHOD, PREC_VALU>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
-            case WIND_SOLVER_MFMA_F32:
-                return launch_dense<METHOD, PREC_MFMA_F32>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
-            case WIND_SOLVER_MFMA_BF16:
-                return launch_dense<METHOD, PREC_MFMA_BF16>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
-            default:
-                return launch_dense<METHOD, PREC_MFMA_BF16X3>(num_cu, d, a, sc.ir_sweeps, lds, T, g, st);
-            }
+    const MassActionPlan plan = mass_action_plan(d, mode);
+    if (plan.dense) {
+        const size_t lds = plan.lds;
+        int T = ((N + 15) & ~15) <= 64 ? 64 : 512;
+        if (sc.block == 64 || sc.block == 128 || sc.block == 256 || sc.block == 512) T = sc.block;
+        const int solver = sc.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : sc.solver;
+        switch (solver) {
+        case WIND_SOLVER_VALU: return launch_dense<METHOD, PREC_VALU>(num_cu, d, a, sc.ir_sweeps, lds, T, sc, st);
+        case WIND_SOLVER_MFMA_F32:
+            return launch_dense<METHOD, PREC_MFMA_F32>(num_cu, d, a, sc.ir_sweeps, lds, T, sc, st);
+        case WIND_SOLVER_MFMA_BF16:
+            return launch_dense<METHOD, PREC_MFMA_BF16>(num_cu, d, a, sc.ir_sweeps, lds, T, sc, st);
+        default:
+            return launch_dense<METHOD, PREC_MFMA_BF16X3>(num_cu, d, a, sc.ir_sweeps, lds, T, sc, st);
         }
     }
     const int T = N <= 64 ? 64 : 256;
-    const size_t base = sizeof(float) * (5 * (size_t)N + 3 * (size_t)d.R);
-    const size_t mats = sizeof(float) * (fast ? 1 : 2) * (size_t)N * N;
-    const bool in_lds = base + mats + 64 <= (size_t)max_lds;
-    const size_t lds = base + (in_lds ? mats : 0);
+    const bool in_lds = plan.in_lds;
+    const size_t lds = plan.lds;
+    int max_lds = 0, dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (lds + 64 > (size_t)max_lds) return wind_fail(WIND_ERR_UNSUPPORTED, "network too large for LDS");
     auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
     if (lds > 65536) {

@@ -46,6 +46,23 @@ struct Pool {
     size_t bytes = 0;
 };
 
+// Autotuned kernel shape per (network, method, mode, controller, solver,
+// refinement sweeps), with its measured cost per system.  Guarded by the
+// context lock.
+struct TuneKey {
+    const wind_network *net;
+    int method, mode, controller, solver, ir;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(net, method, mode, controller, solver, ir) <
+               std::tie(o.net, o.method, o.mode, o.controller, o.solver, o.ir);
+    }
+};
+struct TuneVal {
+    SolverChoice sc;
+    double sec_per_sys = 0.0;
+};
+std::map<TuneKey, TuneVal> g_tuned;
+
 struct Context {
     std::mutex mu;
     bool ready = false;
@@ -257,6 +274,21 @@ int wind_network_create_plugin(const char *path, wind_network **out) {
 
 void wind_network_destroy(wind_network *net) {
     if (!net) return;
+    {
+        // Cached batch state must not outlive the network: the captured graph
+        // holds its device tables, and a later network may reuse the address.
+        Context &c = ctx();
+        std::lock_guard<std::mutex> lock(c.mu);
+        if (c.graph_exec && std::find(c.graph_key.begin(), c.graph_key.end(), (uint64_t)(uintptr_t)net) !=
+                                c.graph_key.end()) {
+            (void)hipDeviceSynchronize();
+            (void)hipGraphExecDestroy(c.graph_exec);
+            c.graph_exec = nullptr;
+            c.graph_key.clear();
+        }
+        for (auto it = g_tuned.begin(); it != g_tuned.end();)
+            it = it->first.net == net ? g_tuned.erase(it) : std::next(it);
+    }
     mass_action_release(net);
     if (net->dl) dlclose(net->dl);
     delete net;
@@ -380,19 +412,6 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
 
 namespace {
 
-struct TuneKey {
-    const wind_network *net;
-    int method, mode, controller, solver, ir;
-    bool operator<(const TuneKey &o) const {
-        return std::tie(net, method, mode, controller, solver, ir) <
-               std::tie(o.net, o.method, o.mode, o.controller, o.solver, o.ir);
-    }
-};
-struct TuneVal {
-    SolverChoice sc;
-    double sec_per_sys = 0.0;
-};
-std::map<TuneKey, TuneVal> g_tuned;
 
 int launch_bucket(const Context &c, const wind_network *net, const IntegrateArgs &a, const wind_options &o,
                   const SolverChoice &sc, hipStream_t st) {
@@ -639,24 +658,8 @@ int wind_launch_async(const wind_network *net, const IntegrateArgs &a, int metho
         int rc = ensure_ready(c);
         if (rc) return rc;
     }
-    if (net->builtin == WIND_NET_MASS_ACTION && mode == WIND_MODE_EXACT) {
-        const int N = net->N;
-        int dev = 0, max_lds = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-        const size_t need = sizeof(float) * (5 * (size_t)N + 3 * (size_t)net->ma.R + 2 * (size_t)N * N) + 64;
-        if (need > (size_t)max_lds)
-            return fail(WIND_ERR_UNSUPPORTED, "exact mode for this network needs a shared scratch buffer");
-    }
-    if (net->builtin == WIND_NET_MASS_ACTION && mode == WIND_MODE_FAST && ((net->N + 15) & ~15) > 192) {
-        const int N = net->N;
-        int dev = 0, max_lds = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-        const size_t need = sizeof(float) * (5 * (size_t)N + 3 * (size_t)net->ma.R + (size_t)N * N) + 64;
-        if (need > (size_t)max_lds)
-            return fail(WIND_ERR_UNSUPPORTED, "this network needs a shared scratch buffer");
-    }
+    if (net->builtin == WIND_NET_MASS_ACTION && mass_action_needs_scratch(net, mode))
+        return fail(WIND_ERR_UNSUPPORTED, "this network's matrices do not fit in LDS (it needs the shared scratch buffer)");
     return method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, mode, kernel, sc, st)
                               : dispatch<WIND_RK2>(c, net, a, mode, kernel, sc, st);
 }

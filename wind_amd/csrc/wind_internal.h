@@ -68,6 +68,10 @@ template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const wind::IntegrateArgs &a, int mode,
                        int kernel, const SolverChoice &sc, hipStream_t st);
 
+// True when launch_mass_action would use the process-wide matrix scratch
+// (matrices too large for LDS): only callers holding the context lock.
+bool mass_action_needs_scratch(const wind_network *net, int mode);
+
 // Enqueues one integration kernel on `st` without synchronising (device
 // pointers in `a`; a.counters must point at 4 zeroed 64-bit counters).
 // Fails with WIND_ERR_UNSUPPORTED for paths that need the shared scratch
