@@ -559,13 +559,19 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
         }
         work += tv[i].sec_per_sys * buckets[i].nsys;
     }
+    // Every bucket gets a full-device persistent grid, and buckets are
+    // launched on their own streams in decreasing cost per system.  The most
+    // expensive bucket starts on every CU; as its queue drains, its workgroups
+    // exit and the dispatcher backfills the freed CUs with the next buckets'
+    // workgroups, so no CU idles while any bucket still has systems.
     std::vector<SolverChoice> sc(nb);
-    for (int i = 0; i < nb; i++) {
-        sc[i] = tv[i].sc;
-        if (buckets[i].nsys == 0 || work <= 0.0) continue;
-        // each bucket's persistent grid gets the fraction of the device its work needs
-        sc[i].share = (float)std::min(1.0, std::max(1.0 / c.num_cu, tv[i].sec_per_sys * buckets[i].nsys / work));
-    }
+    for (int i = 0; i < nb; i++) sc[i] = tv[i].sc;
+    std::vector<int> order;
+    for (int i = 0; i < nb; i++)
+        if (buckets[i].nsys > 0) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return tv[x].sec_per_sys > tv[y].sec_per_sys; });
+    (void)work;
 
     while ((int)c.bucket_streams.size() < nb) {
         hipStream_t s2;
@@ -583,8 +589,7 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
     auto enqueue = [&]() -> int {
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, cnt, nb * 4);
         WIND_HIP(hipEventRecord(c.fork, st));
-        for (int i = 0; i < nb; i++) {
-            if (buckets[i].nsys == 0) continue;
+        for (int i : order) {
             hipStream_t bs = c.bucket_streams[i];
             WIND_HIP(hipStreamWaitEvent(bs, c.fork, 0));
             IntegrateArgs a = bucket_args(buckets[i], tnow, tend, nfix, ABSOLUTE, RELATIVE, o, cnt + 4 * i);
