@@ -179,6 +179,26 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
                    const float *constants, float *equations, int Nsystems, int Neqn,
                    float ABSOLUTE, float RELATIVE, const wind_options *opts);
 
+/* Output-step driver: the reference harness's outer loop
+ * (python/ode_systems/ode_base.py:315-397, runIntegratorOutput) with the
+ * state resident on the device between output steps.  Integrates over
+ * n_output_steps intervals of dt = (tend - tnow) / n_output_steps (times
+ * accumulated in double as the reference does, passed to each interval as
+ * float), each with the initial trial step dt / n_integration_steps.
+ * Outputs (each optional):
+ *   equations_over_time  [(n_output_steps + 1) * Nsystems * Neqn] states at
+ *                        the output times, the initial state first (host, or
+ *                        device with opts->device_pointers);
+ *   times                [n_output_steps + 1];
+ *   nsteps               [n_output_steps] sub-steps per interval (64-bit);
+ *   step_ms              [n_output_steps] device time per interval.
+ * `equations` receives the final state.  opts->steps_out / status_out are
+ * not used; opts->total_steps receives the sum of nsteps. */
+int wind_integrate_outputs(const wind_network *net, double tnow, double tend, int n_output_steps,
+                           int n_integration_steps, const float *constants, float *equations, int Nsystems,
+                           int Neqn, float ABSOLUTE, float RELATIVE, const wind_options *opts,
+                           float *equations_over_time, double *times, int64_t *nsteps, double *step_ms);
+
 /* One bucket of a mixed batch: systems of one network, state resident on
  * the device (device pointers). */
 typedef struct wind_bucket {
@@ -192,8 +212,9 @@ typedef struct wind_bucket {
 } wind_bucket;
 
 /* Integrates every bucket from tnow to tend in one fork/join launch: one
- * persistent kernel per bucket on its own stream, the device shared between
- * buckets in proportion to their estimated work, each bucket's kernel shape
+ * persistent kernel per bucket on its own stream, each with a full-device
+ * grid, launched in decreasing cost per system (the dispatcher backfills the
+ * CUs of buckets that finish), each bucket's kernel shape
  * (workgroup size, solver precision) chosen by an autotuner that times the
  * candidates on a sample of the bucket's systems (cached per network and
  * settings).  opts->method/mode/controller/solver apply to all buckets;

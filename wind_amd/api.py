@@ -27,7 +27,7 @@ STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 EXPORTED = (
     "wind_network_create_builtin", "wind_network_create_mass_action", "wind_network_destroy",
     "wind_network_size", "wind_network_nconst", "wind_network_create_plugin", "wind_options_init",
-    "wind_integrate", "wind_integrate_lockstep",
+    "wind_integrate", "wind_integrate_lockstep", "wind_integrate_outputs",
     "wind_last_error", "wind_version", "wind_integrate_batch", "wind_server_create", "wind_server_destroy",
     "wind_server_block_create", "wind_server_block_read", "wind_server_block_destroy", "wind_server_submit",
     "wind_server_wait", "wind_server_inflight",
@@ -90,6 +90,9 @@ def load(path: str = LIB_PATH):
     lib.wind_options_init.restype = None
     lib.wind_integrate.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(WindOptions)]
+    lib.wind_integrate_outputs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float,
+                                           C.POINTER(WindOptions), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.wind_integrate_lockstep.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_void_p,
                                             C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_int, C.POINTER(C.c_int64)]
@@ -213,6 +216,71 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
     _check(lib.wind_integrate(net.handle, tnow, tend, n_integration_steps, _ptr(constants),
                               _ptr(equations), nsys, net.N, absolute, relative, C.byref(o)))
     return total.value, kms.value
+
+
+def integrate_outputs(net: Network, tnow, tend, n_output_steps, n_integration_steps, constants, equations, nsys,
+                      absolute, relative, method=SIE, mode=MODE_EXACT, kernel=KERNEL_AUTO, const_stride=0,
+                      max_attempts=0, controller=CTRL_REFERENCE, solver=SOLVER_AUTO, ir_sweeps=0):
+    """The reference harness's output loop (runIntegratorOutput) with the state kept on
+    the device.  `equations` (host float32) receives the final state.  Returns
+    (times[n_out + 1], equations_over_time[n_out + 1, nsys * N] float32,
+    nsteps[n_out] int64, step_ms[n_out])."""
+    lib = load()
+    o = WindOptions()
+    lib.wind_options_init(C.byref(o))
+    o.method, o.mode, o.kernel, o.const_stride = method, mode, kernel, const_stride
+    o.controller, o.solver, o.ir_sweeps, o.max_attempts = controller, solver, ir_sweeps, max_attempts
+    for a in (constants, equations):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous):
+            raise WindError("host arrays must be C-contiguous float32 numpy arrays")
+    eot = np.empty((n_output_steps + 1, nsys * net.N), np.float32)
+    times = np.empty(n_output_steps + 1, np.float64)
+    nsteps = np.empty(n_output_steps, np.int64)
+    step_ms = np.empty(n_output_steps, np.float64)
+    _check(lib.wind_integrate_outputs(net.handle, tnow, tend, n_output_steps, n_integration_steps,
+                                      constants.ctypes.data, equations.ctypes.data, nsys, net.N, absolute, relative,
+                                      C.byref(o), eot.ctypes.data, times.ctypes.data, nsteps.ctypes.data,
+                                      step_ms.ctypes.data))
+    return times, eot, nsteps, step_ms
+
+
+def write_ode_cache(path, integrator_name, times, equations_over_time, nsteps, walltimes, attrs=None):
+    """Writes one integrator's results in the layout of the reference's ODE
+    cache (ode_base.py:371-383 and dumpToODECache :295-309, read by odecache.py:90-158):
+    a group named after the integrator with datasets equations_over_time,
+    times, nsteps, walltimes, and the run's attributes on the file.  HDF5 when
+    h5py is importable, else an .npz holding the same names ("<group>/<dataset>",
+    and "attrs" as JSON)."""
+    attrs = dict(attrs or {})
+    try:
+        import h5py  # noqa: F401
+    except ImportError:
+        h5py = None
+    if h5py is not None and not path.endswith(".npz"):
+        with h5py.File(path, "a") as f:
+            if integrator_name in f:
+                del f[integrator_name]
+            g = f.create_group(integrator_name)
+            g["equations_over_time"] = equations_over_time
+            g["times"] = times
+            g["nsteps"] = nsteps
+            g["walltimes"] = walltimes
+            for k, v in attrs.items():
+                f.attrs[k] = v
+        return path
+    import json
+    out = path if path.endswith(".npz") else path + ".npz"
+    data = {}
+    if os.path.exists(out):
+        with np.load(out) as old:
+            data = {k: old[k] for k in old.files if not k.startswith(integrator_name + "/")}
+    data[f"{integrator_name}/equations_over_time"] = np.asarray(equations_over_time)
+    data[f"{integrator_name}/times"] = np.asarray(times)
+    data[f"{integrator_name}/nsteps"] = np.asarray(nsteps)
+    data[f"{integrator_name}/walltimes"] = np.asarray(walltimes)
+    data["attrs"] = np.array(json.dumps({k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in attrs.items()}))
+    np.savez(out, **data)
+    return out
 
 
 def integrate_lockstep(net: Network, tnow, tend, n_integration_steps, constants, equations, nsys,

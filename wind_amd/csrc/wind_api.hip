@@ -76,6 +76,9 @@ struct Context {
     std::vector<hipEvent_t> bucket_events;
     hipEvent_t fork = nullptr;
     Pool tune_state, batch_counters;
+    // output-step driver
+    Pool snapshots, out_counters;
+    std::vector<hipEvent_t> out_events;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
 };
@@ -297,6 +300,141 @@ void wind_network_destroy(wind_network *net) {
 int wind_network_size(const wind_network *net) { return net ? net->N : 0; }
 
 int wind_network_nconst(const wind_network *net) { return net ? net->nconst : 0; }
+
+int wind_integrate_outputs(const wind_network *net, double tnow, double tend, int n_output_steps,
+                           int n_integration_steps, const float *constants, float *equations, int Nsystems,
+                           int Neqn, float ABSOLUTE, float RELATIVE, const wind_options *opts_in,
+                           float *equations_over_time, double *times, int64_t *nsteps, double *step_ms) {
+    g_last_error.clear();
+    wind_options opts;
+    if (opts_in)
+        opts = *opts_in;
+    else
+        wind_options_init(&opts);
+    if (!net) return fail(WIND_ERR_ARGUMENT, "network is NULL");
+    if (opts.method != WIND_SIE && opts.method != WIND_RK2) return fail(WIND_ERR_ARGUMENT, "bad method");
+    if (opts.mode != WIND_MODE_EXACT && opts.mode != WIND_MODE_FAST) return fail(WIND_ERR_ARGUMENT, "bad mode");
+    if (opts.controller != WIND_CTRL_REFERENCE && opts.controller != WIND_CTRL_ADAPTIVE)
+        return fail(WIND_ERR_ARGUMENT, "bad controller");
+    if (opts.solver < WIND_SOLVER_AUTO || opts.solver > WIND_SOLVER_MFMA_BF16X3)
+        return fail(WIND_ERR_ARGUMENT, "unknown solver");
+    if (opts.ir_sweeps < 0 || opts.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
+    if (n_output_steps < 1) return fail(WIND_ERR_ARGUMENT, "n_output_steps < 1");
+    if (n_integration_steps < 0) return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0");
+    if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
+    if (Neqn != net->N) return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network");
+    if (Nsystems > 0 && (!constants || !equations)) return fail(WIND_ERR_ARGUMENT, "NULL state or constants");
+    const int stride = opts.const_stride > 0 ? opts.const_stride : net->nconst;
+    // Output times as the reference's loop forms them (ode_base.py:324-383):
+    // dt = (tend - tnow) / n_output_steps, tcur += dt, in double.
+    const double dt = (tend - tnow) / n_output_steps;
+    {
+        double tc = tnow;
+        if (times) times[0] = tc;
+        for (int i = 0; i < n_output_steps; i++) {
+            tc += dt;
+            if (times) times[i + 1] = tc;
+        }
+    }
+    if (nsteps)
+        for (int i = 0; i < n_output_steps; i++) nsteps[i] = 0;
+    if (step_ms)
+        for (int i = 0; i < n_output_steps; i++) step_ms[i] = 0.0;
+    const size_t eq_floats = (size_t)Nsystems * Neqn;
+    const size_t eq_bytes = eq_floats * sizeof(float);
+    if (Nsystems == 0) return WIND_OK;
+
+    Context &c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    int rc = ensure_ready(c);
+    if (rc) return rc;
+    hipStream_t st = opts.stream ? (hipStream_t)opts.stream : c.stream;
+    const size_t c_bytes = ((size_t)(Nsystems - 1) * stride + net->nconst) * sizeof(float);
+    // Snapshot k (k = 0 .. n_output_steps) of the state lives in slot k of a
+    // device buffer; step k copies slot k - 1 into slot k and integrates it in
+    // place, so the state never leaves the device between output steps.
+    rc = reserve(c.snapshots, eq_bytes * (size_t)(n_output_steps + 1));
+    if (!rc) rc = reserve(c.out_counters, (size_t)n_output_steps * 4 * sizeof(unsigned long long));
+    if (!rc && !opts.device_pointers) rc = reserve(c.constants, c_bytes);
+    if (rc) return rc;
+    while ((int)c.out_events.size() < n_output_steps + 1) {
+        hipEvent_t e;
+        WIND_HIP(hipEventCreate(&e));
+        c.out_events.push_back(e);
+    }
+    float *snap = (float *)c.snapshots.ptr;
+    const float *d_const = constants;
+    if (!opts.device_pointers) {
+        WIND_HIP(hipMemcpyAsync(c.constants.ptr, constants, c_bytes, hipMemcpyHostToDevice, st));
+        WIND_HIP(hipMemcpyAsync(snap, equations, eq_bytes, hipMemcpyHostToDevice, st));
+        d_const = (const float *)c.constants.ptr;
+    } else {
+        WIND_HIP(hipMemcpyAsync(snap, equations, eq_bytes, hipMemcpyDeviceToDevice, st));
+    }
+    unsigned long long *cnt = (unsigned long long *)c.out_counters.ptr;
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, cnt, n_output_steps * 4);
+    SolverChoice sc;
+    sc.solver = opts.solver;
+    sc.ir_sweeps = opts.ir_sweeps;
+    double tcur = tnow;
+    WIND_HIP(hipEventRecord(c.out_events[0], st));
+    for (int k = 1; k <= n_output_steps; k++) {
+        float *slot = snap + eq_floats * (size_t)k;
+        WIND_HIP(hipMemcpyAsync(slot, slot - eq_floats, eq_bytes, hipMemcpyDeviceToDevice, st));
+        const float t0 = (float)tcur, t1 = (float)(tcur + dt);  // ctypes.c_float(tcur), c_float(tcur + dt)
+        tcur += dt;
+        if (t0 < t1) {
+            IntegrateArgs a;
+            a.tnow = t0;
+            a.tend = t1;
+            a.h0 = (t1 - t0) / (float)n_integration_steps;
+            a.ABS = ABSOLUTE;
+            a.REL = RELATIVE;
+            a.controller = opts.controller;
+            a.constants = d_const;
+            a.equations = slot;
+            a.nsys = Nsystems;
+            a.const_stride = stride;
+            a.max_attempts = opts.max_attempts > 0 ? opts.max_attempts : kDefaultMaxAttempts;
+            a.steps_out = nullptr;
+            a.status_out = nullptr;
+            a.index = nullptr;
+            a.counters = cnt + 4 * (k - 1);
+            rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, opts.mode, opts.kernel, sc, st)
+                                         : dispatch<WIND_RK2>(c, net, a, opts.mode, opts.kernel, sc, st);
+            if (rc) return rc;
+        }
+        WIND_HIP(hipEventRecord(c.out_events[k], st));
+    }
+    std::vector<unsigned long long> host_cnt((size_t)n_output_steps * 4);
+    WIND_HIP(hipMemcpyAsync(host_cnt.data(), cnt, host_cnt.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, st));
+    const float *last = snap + eq_floats * (size_t)n_output_steps;
+    if (opts.device_pointers) {
+        WIND_HIP(hipMemcpyAsync(equations, last, eq_bytes, hipMemcpyDeviceToDevice, st));
+        if (equations_over_time)
+            WIND_HIP(hipMemcpyAsync(equations_over_time, snap, eq_bytes * (size_t)(n_output_steps + 1),
+                                    hipMemcpyDeviceToDevice, st));
+    } else {
+        WIND_HIP(hipMemcpyAsync(equations, last, eq_bytes, hipMemcpyDeviceToHost, st));
+        if (equations_over_time)
+            WIND_HIP(hipMemcpyAsync(equations_over_time, snap, eq_bytes * (size_t)(n_output_steps + 1),
+                                    hipMemcpyDeviceToHost, st));
+    }
+    WIND_HIP(hipStreamSynchronize(st));
+    int64_t total = 0;
+    for (int k = 0; k < n_output_steps; k++) {
+        if (nsteps) nsteps[k] = (int64_t)host_cnt[4 * (size_t)k + 1];
+        total += (int64_t)host_cnt[4 * (size_t)k + 1];
+        if (step_ms) {
+            float ms = 0.0f;
+            WIND_HIP(hipEventElapsedTime(&ms, c.out_events[k], c.out_events[k + 1]));
+            step_ms[k] = ms;
+        }
+    }
+    if (opts.total_steps) *opts.total_steps = total;
+    return WIND_OK;
+}
 
 int wind_integrate(const wind_network *net, float tnow, float tend, int n_integration_steps,
                    const float *constants, float *equations, int Nsystems, int Neqn,
