@@ -159,3 +159,35 @@ def test_float_literals_round_trip():
     for v in (1.0, 3.0e-2, 1e-18, 7.5e-7, 123456.789, 2.0**-126):
         lit = codegen._lit(v)
         assert np.float32(float(lit[:-1])) == np.float32(v), lit
+
+
+def test_tiled_network_is_block_diagonal():
+    """demo_decay tiled three times: each tile's RHS and Jacobian block equal
+    the untiled network's on that tile's state, bit for bit, and the
+    Jacobian is zero off the diagonal blocks."""
+    one, three = O.generated("demo_decay"), O.generated("demo_decay_t3")
+    assert three.N == 3 * one.N
+    rng = np.random.default_rng(11)
+    N = one.N
+    for trial in range(10):
+        y = rng.uniform(0.01, 2.0, 3 * N).astype(np.float32)
+        c = rng.uniform(0.5, 2.0, 2).astype(np.float32)
+        for dtype in (np.float32, np.float64):
+            f3, J3 = _eval(three, y, c, dtype)
+            J3 = J3.reshape(3 * N, 3 * N)  # J3[col, row]
+            for t in range(3):
+                f1, J1 = _eval(one, y[t * N:(t + 1) * N], c, dtype)
+                assert np.array_equal(f3[t * N:(t + 1) * N].view(np.uint8), f1.view(np.uint8))
+                blk = J3[t * N:(t + 1) * N, t * N:(t + 1) * N].reshape(-1)
+                assert np.array_equal(blk.view(np.uint8), J1.view(np.uint8))
+            mask = np.kron(np.eye(3), np.ones((N, N))) == 0
+            assert not J3[mask].any()
+
+
+def test_tile_expansion():
+    spec = {"name": "x", "species": 2, "ntile": 2, "nconst": 1,
+            "reactions": [{"reactants": [0, 1], "products": [1], "k": 1.0, "const": 0}]}
+    t = codegen.tile(spec)
+    assert t["species"] == 4 and [r["reactants"] for r in t["reactions"]] == [[0, 1], [2, 3]]
+    assert [r["products"] for r in t["reactions"]] == [[1], [3]]
+    assert codegen.generate(spec)["N"] == 4

@@ -150,3 +150,29 @@ def test_generated_networks_in_a_batch():
     torch.cuda.synchronize()
     for b, ya in zip(bk, alone):
         assert np.array_equal(b["equations"].cpu().numpy(), ya), b["net"].name
+
+
+def test_tiled_network_reproduces_untiled_per_tile():
+    """Identical tiles make identical error tests, so the tiled system takes
+    the untiled system's steps: each tile's final state equals the untiled
+    result bit for bit, with the same sub-step counts; and the tiled network
+    equals its generated oracle statement."""
+    ns = 96
+    rng = np.random.default_rng(4)
+    y1 = rng.uniform(0.1, 1.0, (ns, 4)).astype(np.float32)
+    c = rng.uniform(0.5, 2.0, (ns, 2)).astype(np.float32).reshape(-1)
+    y3 = np.tile(y1, (1, 3)).reshape(-1).copy()
+    s1 = np.zeros(ns, np.int32)
+    s3 = np.zeros(ns, np.int32)
+    ya = y1.reshape(-1).copy()
+    api.integrate(api.Network.plugin("demo_decay"), 0.0, 50.0, 10, c, ya, ns, 1e-4, 1e-4, mode=api.MODE_EXACT,
+                  steps_out=s1, max_attempts=20000)
+    yb = y3.copy()
+    api.integrate(api.Network.plugin("demo_decay_t3"), 0.0, 50.0, 10, c, yb, ns, 1e-4, 1e-4, mode=api.MODE_EXACT,
+                  steps_out=s3, max_attempts=20000)
+    assert np.array_equal(s1, s3)
+    assert same(yb, np.tile(ya.reshape(ns, 4), (1, 3)))
+    yr = y3.copy()
+    O.gold_integrate(api.SIE, O.generated("demo_decay_t3"), 0.0, 50.0, 10, c, yr, ns, 1e-4, 1e-4, const_stride=2,
+                     max_attempts=20000)
+    assert same(yb, yr)

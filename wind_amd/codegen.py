@@ -18,9 +18,15 @@ and rebuilds the libraries.  Here a network description becomes:
 
 Description (JSON or dict):
 
-    {"name": "net", "species": N | [names], "nconst": C,
+    {"name": "net", "species": N | [names], "nconst": C, "ntile": T,
      "reactions": [{"reactants": [a] | [a, b], "products": [c, ...],
                     "k": float, "const": j | null}, ...]}
+
+"ntile" (default 1) repeats the network T times block-diagonally in one
+system of T*N equations, every copy reading the same constants: the
+reference's Ntile tiling (ode_base.py:215-242, the reindexed per-tile code
+blocks of ode_base.py:86-88).  Tile t's species are t*N .. t*N + N-1 and its
+reactions follow tile t-1's.
 
 Reaction rate w = ((k * c[j]) * y_a) * y_b (the factor c[j] only when "const"
 is given, y_b only for two reactants).  Each reactant loses w and each
@@ -57,6 +63,24 @@ def _lit(v: float) -> str:
     if not any(ch in t for ch in ".e"):
         t += ".0"
     return t + "f"
+
+
+def tile(spec) -> dict:
+    """The description with its "ntile" copies expanded into one network."""
+    T = int(spec.get("ntile", 1))
+    if T < 1:
+        raise ValueError("ntile must be >= 1")
+    if T == 1:
+        return spec
+    N = spec["species"] if isinstance(spec["species"], int) else len(spec["species"])
+    out = dict(spec)
+    out["ntile"] = 1
+    out["species"] = N * T
+    out["reactions"] = [
+        {**rx, "reactants": [int(s) + t * N for s in rx["reactants"]],
+         "products": [int(s) + t * N for s in rx.get("products", [])]}
+        for t in range(T) for rx in spec["reactions"]]
+    return out
 
 
 def _check(spec):
@@ -126,6 +150,8 @@ def _body(spec, N, real, cexpr):
 
 def generate(spec) -> dict:
     """Returns {"header", "plugin", "oracle", "N", "nconst", "R", "flops_rhs", "flops_jac"}."""
+    _check(spec)
+    spec = tile(spec)
     name, N, C = _check(spec)
     R = len(spec["reactions"])
     rhs, jac, fr, fj = _body(spec, N, "", lambda j: f"c[{j}]")

@@ -157,14 +157,14 @@ def generated_specs():
     """Descriptions of the networks that the build generates: the mixed
     workload's N = 5 and N = 10 networks (as "mixed5", "mixed10"), and a small
     network with one- and two-body reactions and two constants per system
-    ("demo_decay")."""
+    ("demo_decay"), and that network tiled three times ("demo_decay_t3")."""
     from wind_amd import codegen
 
     specs = []
     for N in GENERATED_MIXED:
         rx, k = mixed_network(N)
         specs.append(codegen.mass_action_spec(f"mixed{N}", N, rx, k))
-    specs.append({
+    demo = {
         "name": "demo_decay", "species": 4, "nconst": 2,
         "reactions": [
             {"reactants": [0], "products": [1], "k": 1.0, "const": 0},
@@ -173,6 +173,8 @@ def generated_specs():
             {"reactants": [2, 3], "products": [0, 1], "k": 2.0e-3, "const": 1},
             {"reactants": [3], "products": [], "k": 1.0e-4, "const": 0},
         ],
-    })
+    }
+    specs.append(demo)
+    specs.append({**demo, "name": "demo_decay_t3", "ntile": 3})
     return specs
 
