@@ -246,8 +246,10 @@ __device__ inline void wave_add_steps(unsigned long long *total, unsigned long l
     if (__lane_id() == 0 && v) atomicAdd(total, v);
 }
 
+// The body of lane_integrate, shared with kernels that set other launch
+// bounds or are compiled with other floating-point contraction.
 template <class Net, int METHOD, class M>
-__global__ void __launch_bounds__(256) lane_integrate(IntegrateArgs a, typename Net::Params p) {
+__device__ inline void lane_integrate_body(const IntegrateArgs &a, const typename Net::Params &p) {
     constexpr int N = Net::N;
     constexpr int NC = Net::NCONST;
     unsigned int *head = reinterpret_cast<unsigned int *>(a.counters);
@@ -289,6 +291,11 @@ __global__ void __launch_bounds__(256) lane_integrate(IntegrateArgs a, typename 
         }
     }
     wave_add_steps(a.counters + 1, steps);
+}
+
+template <class Net, int METHOD, class M>
+__global__ void __launch_bounds__(256) lane_integrate(IntegrateArgs a, typename Net::Params p) {
+    lane_integrate_body<Net, METHOD, M>(a, p);
 }
 
 // ---------------------------------------------------------------------------

@@ -141,11 +141,7 @@ int launch_small(const Context &c, const wind_network *net, const IntegrateArgs 
         if (rc) return rc;
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     } else {
-        auto k = lane_integrate<Net, METHOD, MathFast>;
-        int grid = 0;
-        int rc = persistent_grid(c, k, 256, a.nsys, &grid, sc);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
+        return launch_lane_fast<Net, METHOD>(c.num_cu, p, a, sc, st);
     }
     WIND_HIP(hipGetLastError());
     return WIND_OK;

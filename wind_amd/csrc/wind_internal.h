@@ -68,6 +68,12 @@ template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const wind::IntegrateArgs &a, int mode,
                        int kernel, const SolverChoice &sc, hipStream_t st);
 
+// Fast-mode lane kernels of the compiled-in networks (lane_fast.hip,
+// compiled with fused multiply-add contraction).
+template <class Net, int METHOD>
+int launch_lane_fast(int num_cu, const typename Net::Params &p, const wind::IntegrateArgs &a, const SolverChoice &sc,
+                     hipStream_t st);
+
 // True when launch_mass_action would use the process-wide matrix scratch
 // (matrices too large for LDS): only callers holding the context lock.
 bool mass_action_needs_scratch(const wind_network *net, int mode);
