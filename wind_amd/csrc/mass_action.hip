@@ -91,24 +91,6 @@ void mass_action_release(wind_network *n) {
     d = MassActionDevice();
 }
 
-template <int METHOD, int PREC>
-int launch_dense(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, int ir, size_t lds, int T,
-                 const SolverChoice &sc, hipStream_t st) {
-    auto kfn = ma_dense_integrate<METHOD, PREC>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    }
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, T, lds);
-    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys);
-    hipLaunchKernelGGL(kfn, dim3(grid), dim3(T), lds, st, a, d, ir);
-    e = hipGetLastError();
-    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    return WIND_OK;
-}
-
 // Which kernel launch_mass_action picks, and whether it needs the
 // process-wide matrix scratch (only callers holding the context lock may
 // use that).

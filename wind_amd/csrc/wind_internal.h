@@ -68,6 +68,12 @@ template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const wind::IntegrateArgs &a, int mode,
                        int kernel, const SolverChoice &sc, hipStream_t st);
 
+// Dense LDS/MFMA solver launch (dense.hip); PREC is a kernels_dense.h
+// DensePrec value.
+template <int METHOD, int PREC>
+int launch_dense(int num_cu, const wind::MassActionDevice &d, const wind::IntegrateArgs &a, int ir, size_t lds,
+                 int T, const SolverChoice &sc, hipStream_t st);
+
 // Fast-mode lane kernels of the compiled-in networks (lane_fast.hip,
 // compiled with fused multiply-add contraction).
 template <class Net, int METHOD>
