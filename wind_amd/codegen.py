@@ -243,7 +243,7 @@ def _write(path, text):
 
 def build(spec, oracle: bool = True, verbose: bool = False) -> str:
     """Generates and compiles a network; returns the plugin library's path."""
-    from wind_amd.build import HIP_FLAGS, HIPCC, LIB
+    from wind_amd.build import ARCH, HIP_FLAGS, HIPCC, LIB
 
     g = generate(spec)
     name = g["name"]
@@ -255,8 +255,16 @@ def build(spec, oracle: bool = True, verbose: bool = False) -> str:
         json.dump({"name": name, "N": g["N"], "nconst": g["nconst"], "R": g["R"], "flops_rhs": g["flops_rhs"],
                    "flops_jac": g["flops_jac"]}, f)
     plugin = os.path.join(d, "libwind_net.so")
-    jobs = [[HIPCC, *HIP_FLAGS, "-shared", f"-I{CSRC}", f"-I{src}", os.path.join(src, f"plugin_{name}.hip"),
-             "-o", plugin + ".tmp"]]
+    psrc = os.path.join(src, f"plugin_{name}.hip")
+    part1, part2 = os.path.join(src, "part1.o"), os.path.join(src, "part2.o")
+    # part 1: exact mode, block and lockstep kernels, the descriptor (no FMA
+    # contraction); part 2: fast-mode lane kernels (FMA contraction).
+    compile_jobs = [
+        [HIPCC, *HIP_FLAGS, "-DWIND_PLUGIN_PART=1", f"-I{CSRC}", f"-I{src}", "-c", psrc, "-o", part1],
+        [HIPCC, *HIP_FLAGS, "-ffp-contract=fast", "-DWIND_PLUGIN_PART=2", f"-I{CSRC}", f"-I{src}", "-c", psrc,
+         "-o", part2],
+    ]
+    jobs = []
     for kind, lib in ((1, "sie.so"), (2, "rk2.so"), (3, "sie_host.so")):
         jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-DWIND_SHIM_KIND={kind}",
                      "-DWIND_SHIM_NETWORK=4", f"-DWIND_SHIM_NUM_CONST={g['nconst']}",
@@ -268,7 +276,8 @@ def build(spec, oracle: bool = True, verbose: bool = False) -> str:
         jobs.append(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", osrc,
                      "-o", os.path.join(ORACLE_GEN, f"{name}.so")])
     with ThreadPoolExecutor(max_workers=4) as ex:
-        list(ex.map(_run, jobs[1:] + jobs[:1]))
+        list(ex.map(_run, compile_jobs + jobs))
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", part1, part2, "-o", plugin + ".tmp"])
     os.replace(plugin + ".tmp", plugin)
     if verbose:
         print("built network", name, "->", plugin)

@@ -3,6 +3,9 @@
 //     #include "<network header>"
 //     #include "plugin_impl.h"
 //     WIND_DEFINE_PLUGIN(NetType, "name", nreactions, flops_rhs, flops_jac)
+// and is compiled twice into the plugin: with WIND_PLUGIN_PART=1 (exact
+// mode, block and lockstep kernels, the descriptor; no FMA contraction) and
+// WIND_PLUGIN_PART=2 (the fast-mode lane kernels, with FMA contraction).
 #pragma once
 #include "kernels_lockstep.h"
 #include "kernels_small.h"
@@ -23,6 +26,14 @@ inline int plugin_grid(K kernel, int block, int nsys, int num_cu, float share, i
     return WIND_OK;
 }
 
+#ifndef WIND_PLUGIN_PART
+#define WIND_PLUGIN_PART 1
+#endif
+
+extern "C" __attribute__((visibility("hidden"))) int wind_plugin_launch_fast(const IntegrateArgs *a, int method,
+                                                                          int num_cu, float share, int grid,
+                                                                          hipStream_t st);
+
 template <class Net, int METHOD>
 int plugin_launch_method(const IntegrateArgs &a, int mode, int kernel, int num_cu, float share, int grid_cap,
                          hipStream_t st) {
@@ -41,11 +52,19 @@ int plugin_launch_method(const IntegrateArgs &a, int mode, int kernel, int num_c
         if (rc) return rc;
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     } else {
-        auto k = lane_integrate<Net, METHOD, MathFast>;
-        rc = plugin_grid(k, 256, a.nsys, num_cu, share, grid_cap, &grid);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
+        return wind_plugin_launch_fast(&a, METHOD, num_cu, share, grid_cap, st);
     }
+    return hipGetLastError() == hipSuccess ? WIND_OK : WIND_ERR_DEVICE;
+}
+
+template <class Net, int METHOD>
+int plugin_launch_fast_method(const IntegrateArgs &a, int num_cu, float share, int grid_cap, hipStream_t st) {
+    typename Net::Params p{};
+    int grid = 0;
+    auto k = lane_integrate<Net, METHOD, MathFast>;
+    const int rc = plugin_grid(k, 256, a.nsys, num_cu, share, grid_cap, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     return hipGetLastError() == hipSuccess ? WIND_OK : WIND_ERR_DEVICE;
 }
 
@@ -68,6 +87,7 @@ int plugin_lockstep(float tnow, float hh, const float *C, int stride, const floa
 
 }  // namespace wind
 
+#if WIND_PLUGIN_PART == 1
 #define WIND_DEFINE_PLUGIN(NET, NAME, NREACT, FLOPS_RHS, FLOPS_JAC)                                      \
     extern "C" __attribute__((visibility("default"))) const wind_plugin_desc *wind_plugin_get(void) {     \
         static const wind_plugin_desc d = {WIND_PLUGIN_ABI,       NET::N,                                  \
@@ -77,3 +97,15 @@ int plugin_lockstep(float tnow, float hh, const float *C, int stride, const floa
                                            wind::plugin_lockstep<NET>};                                    \
         return &d;                                                                                         \
     }
+#else
+#undef WIND_DEFINE_PLUGIN
+#define WIND_DEFINE_PLUGIN(NET, NAME, NREACT, FLOPS_RHS, FLOPS_JAC)                                      \
+    extern "C" __attribute__((visibility("hidden"))) int wind_plugin_launch_fast(                         \
+        const wind::IntegrateArgs *a, int method, int num_cu, float share, int grid, hipStream_t st) {       \
+        if (method == WIND_SIE)                                                                            \
+            return wind::plugin_launch_fast_method<NET, WIND_SIE>(*a, num_cu, share, grid, st);           \
+        if (method == WIND_RK2)                                                                            \
+            return wind::plugin_launch_fast_method<NET, WIND_RK2>(*a, num_cu, share, grid, st);           \
+        return WIND_ERR_ARGUMENT;                                                                          \
+    }
+#endif
