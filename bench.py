@@ -399,7 +399,7 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": algo_bytes / kernel_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
         "frac": algo_bytes / kernel_s / 1e9 / PEAK_HBM_GBS, "traffic": None,
-        "kernel": "lane_integrate<NetHHe6, SIE>", "kernel_ms": kernel_s * 1e3,
+        "kernel": ("lane_fast_kernel<NetHHe6, SIE>" if mode == api.MODE_FAST else "lane_integrate<NetHHe6, SIE, MathExact>"), "kernel_ms": kernel_s * 1e3,
         "valu": {"achieved": algo_flops / kernel_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                  "frac": algo_flops / kernel_s / 1e12 / PEAK_FP32_TFLOPS, "flops_per_substep": F_SUBSTEP},
     }
