@@ -227,6 +227,36 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
     return false;
 }
 
+// A system's state in and out of HBM.  For even N the system's first float
+// is 8-byte aligned (equations[s*N] with s*N even), so 8-byte vector
+// accesses halve the memory instructions, and each one covers contiguous
+// bytes: fewer partially written lines in the system-major layout.
+template <int N>
+__device__ inline void load_state(const float *ys, float (&y)[N]) {
+    if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 2; i++) {
+            const float2 v = reinterpret_cast<const float2 *>(ys)[i];
+            y[2 * i] = v.x;
+            y[2 * i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) y[i] = ys[i];
+    }
+}
+
+template <int N>
+__device__ inline void store_state(float *ys, const float (&y)[N]) {
+    if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 2; i++) reinterpret_cast<float2 *>(ys)[i] = make_float2(y[2 * i], y[2 * i + 1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) ys[i] = y[i];
+    }
+}
+
 // Ranks the lanes that set `need` and hands each the next queue index.
 __device__ inline int queue_fetch(unsigned int *head, bool need) {
     const unsigned long long mask = __ballot(need);
@@ -267,8 +297,7 @@ __device__ inline void lane_integrate_body(const IntegrateArgs &a, const typenam
                 sid = a.index ? a.index[got] : got;
                 const float *ys = a.equations + (size_t)sid * N;
                 const float *cs = a.constants + (size_t)sid * a.const_stride;
-#pragma unroll
-                for (int i = 0; i < N; i++) s.y[i] = ys[i];
+                load_state<N>(ys, s.y);
 #pragma unroll
                 for (int k = 0; k < NC; k++) c[k] = cs[k];
                 s.tnow = a.tnow;
@@ -281,8 +310,7 @@ __device__ inline void lane_integrate_body(const IntegrateArgs &a, const typenam
         if (!__any(active)) break;
         if (active && attempt<Net, METHOD, M>(p, a, c, s)) {
             float *ys = a.equations + (size_t)sid * N;
-#pragma unroll
-            for (int i = 0; i < N; i++) ys[i] = s.y[i];
+            store_state<N>(ys, s.y);
             if (a.steps_out) a.steps_out[sid] = 3 * s.attempts;
             if (a.status_out) a.status_out[sid] = s.status;
             steps += 3ull * (unsigned long long)s.attempts;
