@@ -227,6 +227,40 @@ def dense_roofline(torch, api, W, nsys):
             "frac": ach / PEAK_FP32_TFLOPS}
 
 
+def dense_precision(torch, api, W, nsys):
+    """The dense solver's precision forms on the N = 157 workload: FP32 MFMA,
+    BF16-input MFMA + 2 refinement sweeps (FP64 residual), split-BF16 (x3) + 1
+    sweep.  Throughput on the first `nsys` systems; accuracy on the 32 systems
+    with FP64 truth (tests/golden/truth_mass_action.npz)."""
+    dev = torch.device("cuda:0")
+    rx, k, y0, sc = W.dense_workload(W.DENSE_NSYS)
+    net = api.Network.mass_action(W.DENSE_N, rx, k)
+    tp = os.path.join(ROOT, "tests", "golden", "truth_mass_action.npz")
+    T = np.load(tp) if os.path.exists(tp) else None
+    out = {}
+    for name, solver, ir in (("fp32_mfma", api.SOLVER_MFMA_F32, 0), ("bf16_mfma_ir2", api.SOLVER_MFMA_BF16, 2),
+                             ("bf16x3_mfma_ir1", api.SOLVER_MFMA_BF16X3, 1)):
+        y = torch.from_numpy(y0[: nsys * W.DENSE_N]).to(dev)
+        c = torch.from_numpy(sc[:nsys]).to(dev)
+        tot, kms = api.integrate(net, 0.0, W.MASS_ACTION_TEND, 1, c, y, nsys, MA_ABS, MA_REL, mode=api.MODE_FAST,
+                                 device_pointers=True, controller=api.CTRL_ADAPTIVE, solver=solver, ir_sweeps=ir,
+                                 want_kernel_ms=True, max_attempts=100000)
+        r = {"systems_per_s": nsys / (kms / 1e3), "substeps": int(tot), "ir_sweeps": ir}
+        if T is not None:
+            sel = T["dense_157_idx"]
+            ya = y0.reshape(-1, W.DENSE_N)[sel].reshape(-1).copy()
+            api.integrate(net, 0.0, float(T["tend"]), 1, sc[sel].copy(), ya, len(sel), MA_ABS, MA_REL,
+                          mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE, solver=solver, ir_sweeps=ir,
+                          max_attempts=100000)
+            e = ma_rel_err(ya.reshape(-1, W.DENSE_N).astype(np.float64), T["dense_157_y"])
+            r["frac_rel_err_le_1e-3"] = float(np.mean(e <= 1e-3))
+            r["median_rel_err"] = float(np.median(e))
+            r["max_rel_err"] = float(e.max())
+        out[name] = r
+    return {"workload": f"N = 157 mass-action, first {nsys} systems (accuracy: the 32 FP64-truth systems)",
+            "forms": out}
+
+
 # ------------------------------------------------------------------ serving
 def serving(api, W, seconds, load_frac, seed=5):
     """Continuous-batching request server: resident cell blocks of mixed
@@ -452,6 +486,7 @@ def main():
     mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain,
                                                 not a.no_generated)
     dense = None if a.no_mixed else dense_roofline(torch, api, W, a.dense_nsys)
+    precision = None if a.no_mixed else dense_precision(torch, api, W, 1024)
     serve = None if a.no_serve else serving(api, W, a.serve_seconds, a.serve_load)
     cpu = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, REF_TOL, 0)
     cpu_same = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, tol, ctrl)
@@ -471,7 +506,7 @@ def main():
         "accuracy": acc, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
         "reference_algorithm": ref_alg, "plain_gpu": plain,
         "cpu_baseline": cpu, "cpu_same_algorithm": cpu_same,
-        "mixed_batch": mixed, "dense_157": dense, "serving": serve,
+        "mixed_batch": mixed, "dense_157": dense, "dense_157_precision": precision, "serving": serve,
     }
     print(json.dumps(line))
 
