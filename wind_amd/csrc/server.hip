@@ -131,6 +131,7 @@ struct wind_server {
     std::unordered_map<int64_t, Request> reqs;
     int64_t next_id = 1;
     int num_cu = 0;
+    int device = 0;  // the device current when the server was created
     std::string last_error;
     std::vector<wind_block *> blocks;  // live resident blocks (freed by destroy)
 };
@@ -232,7 +233,7 @@ bool start(wind_server *srv, Request &r, int si) {
 }
 
 void scheduler(wind_server *srv) {
-    (void)hipSetDevice(0);
+    (void)hipSetDevice(srv->device);
     std::unique_lock<std::mutex> lk(srv->mu);
     while (!srv->stop || !srv->pending.empty() || std::any_of(srv->slots.begin(), srv->slots.end(),
                                                                [](const Slot &s) { return s.request >= 0; })) {
@@ -303,6 +304,7 @@ int wind_server_create(size_t hbm_budget_bytes, int nslots, wind_server **out) {
     if (rc) return rc;
     wind_server *srv = new wind_server();
     srv->num_cu = ncu;
+    (void)hipGetDevice(&srv->device);
     if (hipMalloc((void **)&srv->arena_base, hbm_budget_bytes) != hipSuccess) {
         delete srv;
         return wind_fail(WIND_ERR_DEVICE, "cannot allocate the server arena");
