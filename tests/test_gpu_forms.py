@@ -194,6 +194,30 @@ def test_batch_graph_replays():
             assert np.array_equal(b["equations"].cpu().numpy(), ye), (rep, b["N"])
 
 
+def test_batch_scratch_buckets_match_single_calls():
+    """Exact mode keeps two N x N matrices per system; at N = 144 they exceed
+    LDS and live in the process-wide scratch.  Two such buckets in one batch
+    (with a small in-LDS bucket beside them) must not share it concurrently:
+    every result equals its single call bit for bit."""
+    torch = torch_cuda()
+    dev = torch.device("cuda:0")
+    bk, alone = [], []
+    for N, ns, seed in ((144, 24, 1), (144, 40, 2), (10, 300, 3)):
+        rx, k = W.mass_action_network(N, seed=900 + seed)
+        net = api.Network.mass_action(N, rx, k)
+        y0, sc = W.mass_action_systems(ns, N, seed=seed, scale_range=W.MIXED_SCALE_RANGE)
+        y = y0.copy()
+        api.integrate(net, 0.0, 1e3, 1, sc, y, ns, 1e-3, 1e-3, mode=api.MODE_EXACT, max_attempts=20000)
+        alone.append(y)
+        t0 = torch.from_numpy(y0).to(dev)
+        bk.append(dict(net=net, nsys=ns, equations=t0, constants=torch.from_numpy(sc).to(dev)))
+    api.integrate_batch(bk, 0.0, 1e3, 1, 1e-3, 1e-3, mode=api.MODE_EXACT, controller=api.CTRL_REFERENCE,
+                        max_attempts=20000)
+    torch.cuda.synchronize()
+    for b, ya in zip(bk, alone):
+        assert np.array_equal(b["equations"].cpu().numpy(), ya), b["net"].N
+
+
 def _requests(seed=3, n=12):
     rng = np.random.default_rng(seed)
     reqs = []

@@ -4,19 +4,20 @@
 // 16), factored by a blocked right-looking LU without pivoting (the
 // reference's solve is unpivoted too, gje_gold.c:56-134).
 //
-// Per 16-column panel one wave factors the 16x16 diagonal block and inverts
-// its two triangular factors; the panel blocks L21 = A21 Uinv and
-// U12 = Linv A12 and the trailing update A22 -= L21 U12 are 16x16 tile
-// products spread over the workgroup's waves.  The tile products run on the
-// matrix cores:
+// Per 16-column panel one wave inverts the 16x16 diagonal block in place
+// (G = A11^-1, Gauss-Jordan in registers); the panel blocks W = G A12 and
+// the trailing update A22 -= A21 W are 16x16 tile products spread over the
+// workgroup's waves, and the next diagonal block is updated and inverted
+// while the other waves finish the trailing update.  The tile products run
+// on the matrix cores:
 //   PREC_VALU      fused multiply-adds on the vector ALU (no MFMA), reference point
 //   PREC_MFMA_F32  v_mfma_f32_16x16x4f32, four per tile (K = 16)
 //   PREC_MFMA_BF16 v_mfma_f32_16x16x16bf16_1k, one per tile: BF16 inputs, FP32 accumulation
 //   PREC_MFMA_BF16X3 three BF16 products (hi*hi + hi*lo + lo*hi) of the split operands
 // (the panel products always use FP32: BF16 applies to the trailing update).
-// The triangular solves run in one wave with the right-hand side in
-// registers, one 16-row block at a time through the stored diagonal-block
-// inverses.  Optional iterative refinement recomputes the residual
+// The forward sweep of the solve rides along with the factorisation; the
+// backward sweep runs in one wave with the right-hand side in registers, one
+// 16-row block at a time.  Optional iterative refinement recomputes the residual
 // r = f - (I - hJ)x in double (J x gathered row by row) and solves for a
 // correction with the same factors.  Every step is deterministic: a
 // system's result does not depend on what else runs on the device.

@@ -127,6 +127,69 @@ bool mass_action_needs_scratch(const wind_network *net, int mode) {
     return !p.dense && !p.in_lds;
 }
 
+namespace {
+
+// Launch shape of the one-workgroup-per-system kernel (ma_block_integrate).
+struct BlockShape {
+    int T = 0, grid = 0;
+    size_t lds = 0;
+    bool in_lds = true;
+};
+
+template <int METHOD>
+int block_shape(int num_cu, const MassActionDevice &d, const MassActionPlan &plan, bool fast, const SolverChoice &sc,
+                int nsys, BlockShape *s) {
+    s->T = d.N <= 64 ? 64 : 256;
+    s->in_lds = plan.in_lds;
+    s->lds = plan.lds;
+    int max_lds = 0, dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (s->lds + 64 > (size_t)max_lds) return wind_fail(WIND_ERR_UNSUPPORTED, "network too large for LDS");
+    auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
+    if (s->lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)s->lds);
+        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, s->T, s->lds);
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    s->grid = scaled_grid(num_cu, per_cu, sc, nsys);
+    return WIND_OK;
+}
+
+size_t scratch_bytes(const MassActionDevice &d, const BlockShape &s) {
+    return s.in_lds ? 0 : sizeof(float) * 2 * (size_t)d.N * d.N * s.grid;
+}
+
+}  // namespace
+
+template <int METHOD>
+int mass_action_scratch_bytes(int num_cu, const wind_network *net, int mode, const SolverChoice &sc, int nsys,
+                              size_t *bytes) {
+    *bytes = 0;
+    if (net->builtin != WIND_NET_MASS_ACTION) return WIND_OK;
+    const MassActionPlan plan = mass_action_plan(net->ma, mode);
+    if (plan.dense || plan.in_lds) return WIND_OK;
+    BlockShape s;
+    const int rc = block_shape<METHOD>(num_cu, net->ma, plan, mode == WIND_MODE_FAST, sc, nsys, &s);
+    if (rc) return rc;
+    *bytes = scratch_bytes(net->ma, s);
+    return WIND_OK;
+}
+
+int mass_action_scratch_reserve(size_t need) {
+    if (need <= g_scratch.bytes) return WIND_OK;
+    if (g_scratch.ptr) (void)hipFree(g_scratch.ptr);
+    g_scratch.ptr = nullptr;
+    g_scratch.bytes = 0;
+    hipError_t e = hipMalloc((void **)&g_scratch.ptr, need);
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    g_scratch.bytes = need;
+    return WIND_OK;
+}
+
 template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
                        int kernel, const SolverChoice &sc, hipStream_t st) {
@@ -151,38 +214,22 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
             return launch_dense<METHOD, PREC_MFMA_BF16X3>(num_cu, d, a, sc.ir_sweeps, lds, T, sc, st);
         }
     }
-    const int T = N <= 64 ? 64 : 256;
-    const bool in_lds = plan.in_lds;
-    const size_t lds = plan.lds;
-    int max_lds = 0, dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    if (lds + 64 > (size_t)max_lds) return wind_fail(WIND_ERR_UNSUPPORTED, "network too large for LDS");
-    auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
-        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    }
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, T, lds);
-    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys);
+    BlockShape s;
+    int rc = block_shape<METHOD>(num_cu, d, plan, fast, sc, a.nsys, &s);
+    if (rc) return rc;
+    // Matrices beyond LDS live in the process-wide scratch, one slice per
+    // workgroup.  Launches that use it must not overlap: callers hold the
+    // context lock, and batches reserve it up front and put every bucket
+    // that uses it on one stream.
     float *scratch = nullptr;
-    if (!in_lds) {
-        const size_t need = sizeof(float) * 2 * (size_t)N * N * grid;
-        if (need > g_scratch.bytes) {
-            if (g_scratch.ptr) (void)hipFree(g_scratch.ptr);
-            g_scratch.ptr = nullptr;
-            g_scratch.bytes = 0;
-            e = hipMalloc((void **)&g_scratch.ptr, need);
-            if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-            g_scratch.bytes = need;
-        }
+    if (!s.in_lds) {
+        rc = mass_action_scratch_reserve(scratch_bytes(d, s));
+        if (rc) return rc;
         scratch = g_scratch.ptr;
     }
-    hipLaunchKernelGGL(kfn, dim3(grid), dim3(T), lds, st, a, d, scratch, in_lds ? 1 : 0);
-    e = hipGetLastError();
+    auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
+    hipLaunchKernelGGL(kfn, dim3(s.grid), dim3(s.T), s.lds, st, a, d, scratch, s.in_lds ? 1 : 0);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
     return WIND_OK;
 }
@@ -191,3 +238,5 @@ template int launch_mass_action<WIND_SIE>(int, const wind_network *, const Integ
                                           const SolverChoice &, hipStream_t);
 template int launch_mass_action<WIND_RK2>(int, const wind_network *, const IntegrateArgs &, int, int,
                                           const SolverChoice &, hipStream_t);
+template int mass_action_scratch_bytes<WIND_SIE>(int, const wind_network *, int, const SolverChoice &, int, size_t *);
+template int mass_action_scratch_bytes<WIND_RK2>(int, const wind_network *, int, const SolverChoice &, int, size_t *);

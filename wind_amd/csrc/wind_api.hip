@@ -679,9 +679,8 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
     if (rc) return rc;
     hipStream_t st = o.stream ? (hipStream_t)o.stream : c.stream;
 
-    // Tune each bucket (cached), then split the device by estimated work.
+    // Tune each bucket (cached).
     std::vector<TuneVal> tv(nb);
-    double work = 0.0;
     for (int i = 0; i < nb; i++) {
         if (buckets[i].nsys == 0) continue;
         if (o.flags & WIND_FLAG_NO_AUTOTUNE) {
@@ -691,7 +690,6 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
             rc = tune_bucket(c, buckets[i], tnow, tend, nfix, ABSOLUTE, RELATIVE, o, st, &tv[i]);
             if (rc) return rc;
         }
-        work += tv[i].sec_per_sys * buckets[i].nsys;
     }
     // Every bucket gets a full-device persistent grid, and buckets are
     // launched on their own streams in decreasing cost per system.  The most
@@ -705,7 +703,28 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
         if (buckets[i].nsys > 0) order.push_back(i);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return tv[x].sec_per_sys > tv[y].sec_per_sys; });
-    (void)work;
+    // Buckets whose matrices exceed LDS share the process-wide scratch: it is
+    // grown here, before anything is enqueued or captured, and those buckets
+    // run one after another on the stream of the first of them.
+    std::vector<int> stream_of(nb);
+    {
+        size_t need = 0;
+        int first = -1;
+        for (int i : order) {
+            size_t b = 0;
+            rc = o.method == WIND_SIE
+                     ? mass_action_scratch_bytes<WIND_SIE>(c.num_cu, buckets[i].net, o.mode, sc[i], buckets[i].nsys, &b)
+                     : mass_action_scratch_bytes<WIND_RK2>(c.num_cu, buckets[i].net, o.mode, sc[i], buckets[i].nsys, &b);
+            if (rc) return rc;
+            if (b > 0 && first < 0) first = i;
+            stream_of[i] = b > 0 ? first : i;
+            need = std::max(need, b);
+        }
+        if (need > 0) {
+            rc = mass_action_scratch_reserve(need);
+            if (rc) return rc;
+        }
+    }
 
     while ((int)c.bucket_streams.size() < nb) {
         hipStream_t s2;
@@ -724,7 +743,7 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, cnt, nb * 4);
         WIND_HIP(hipEventRecord(c.fork, st));
         for (int i : order) {
-            hipStream_t bs = c.bucket_streams[i];
+            hipStream_t bs = c.bucket_streams[stream_of[i]];
             WIND_HIP(hipStreamWaitEvent(bs, c.fork, 0));
             IntegrateArgs a = bucket_args(buckets[i], tnow, tend, nfix, ABSOLUTE, RELATIVE, o, cnt + 4 * i);
             int r = launch_bucket(c, buckets[i].net, a, o, sc[i], bs);
@@ -763,12 +782,16 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
                 (void)hipGraphExecDestroy(c.graph_exec);
                 c.graph_exec = nullptr;
             }
-            hipGraph_t g;
-            WIND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+            hipGraph_t g = nullptr;
+            // Thread-local capture: other threads of the process (a server's
+            // scheduler) may keep calling the runtime meanwhile.
+            WIND_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             rc = enqueue();
             hipError_t ec = hipStreamEndCapture(st, &g);
-            if (rc) return rc;
-            if (ec != hipSuccess) return fail(WIND_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+            if (rc || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : fail(WIND_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+            }
             WIND_HIP(hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0));
             (void)hipGraphDestroy(g);
             c.graph_key = key;

@@ -83,6 +83,13 @@ int launch_lane_fast(int num_cu, const typename Net::Params &p, const wind::Inte
 // True when launch_mass_action would use the process-wide matrix scratch
 // (matrices too large for LDS): only callers holding the context lock.
 bool mass_action_needs_scratch(const wind_network *net, int mode);
+// Scratch bytes such a launch needs (0 when it uses none), and growing the
+// scratch to a given size ahead of launches (batches reserve before they
+// enqueue or capture anything).
+template <int METHOD>
+int mass_action_scratch_bytes(int num_cu, const wind_network *net, int mode, const SolverChoice &sc, int nsys,
+                              size_t *bytes);
+int mass_action_scratch_reserve(size_t bytes);
 
 // Enqueues one integration kernel on `st` without synchronising (device
 // pointers in `a`; a.counters must point at 4 zeroed 64-bit counters).
