@@ -227,8 +227,8 @@ int wind_integrate_batch(const wind_bucket *buckets, int nbuckets, float tnow, f
  * A server owns `nslots` execution slots (streams) and a device arena of
  * `hbm_budget_bytes`.  Submitted requests are admitted in arrival order when
  * a slot is free and their footprint (constants + state) fits in the arena,
- * then copied in, integrated (each running kernel takes 1/nslots of the
- * device, so admitted requests overlap), and copied out.  A resident block
+ * then copied in, integrated (each kernel has a full-device persistent grid;
+ * admitted requests overlap as earlier ones drain), and copied out.  A resident block
  * keeps a cell block's state in the arena across requests. */
 typedef struct wind_server wind_server;
 typedef struct wind_block wind_block;

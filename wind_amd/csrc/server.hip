@@ -4,8 +4,10 @@
 // each) and a device arena of a fixed HBM budget.  Requests are admitted in
 // arrival order when a slot is free and their device footprint fits in the
 // arena; each admitted request is copied in, integrated by a persistent
-// kernel that takes 1/nslots of a full-device grid (so admitted requests run
-// concurrently), and copied out, all on its slot's stream.  Completion is
+// kernel with a full-device grid, and copied out, all on its slot's stream.
+// Admitted requests run concurrently: a request's workgroups exit as its
+// queue drains, and the hardware dispatcher hands the freed CUs to the
+// workgroups of requests launched after it.  Completion is
 // observed with an event; latency runs from submission to results on the
 // host.  Resident blocks keep a cell block's state in the arena across
 // requests (only results travel); requests on one block run in order.
@@ -210,8 +212,9 @@ bool start(wind_server *srv, Request &r, int si) {
     a.status_out = nullptr;
     a.index = nullptr;
     a.counters = s.counters;
+    // Full-device grid (measured against a fixed 1/nslots share of the
+    // device: more requests per second and lower p50 and p99 latency).
     SolverChoice sc;
-    sc.share = 1.0f / (float)srv->slots.size();
     if (nsys > 0 && r.d.tnow < r.d.tend) {
         int rc = wind_launch_async(net, a, r.d.method, r.d.mode, WIND_KERNEL_AUTO, sc, s.stream);
         if (rc) {
