@@ -75,29 +75,35 @@ def accuracy(yf, nsys):
 
 
 def cpu_baseline(params, y0, nH, seconds, tol, controller):
-    """The oracle (CPU restatement of c_baseline) on this host's cores over a
-    bounded sample of the same workload."""
+    """The oracle (CPU restatement of c_baseline) on this host's cores over
+    bounded samples of the same workload: all cores (one contiguous slice of
+    systems per thread, median of 3 runs) and one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
     net = O.hhe6(params)
     threads = max(1, min(16, os.cpu_count() or 1))
-    # calibrate on 32 systems, then size the sample for about `seconds`
-    cal = 32
-    y = y0[: cal * N_SPECIES].copy()
-    t = time.perf_counter()
-    O.gold_integrate(O.SIE, net, 0.0, 1.0e6, 1, nH[:cal].copy(), y, cal, tol, tol, const_stride=1,
-                     controller=controller)
-    per_sys = (time.perf_counter() - t) / cal
-    n = int(max(threads * 8, min(NSYS, seconds * threads / max(per_sys, 1e-9))))
+
+    def run(n, nthreads):
+        y = y0[: n * N_SPECIES].copy()
+        t = time.perf_counter()
+        O.gold_integrate(O.SIE, net, 0.0, 1.0e6, 1, nH[:n].copy(), y, n, tol, tol, const_stride=1, threads=nthreads,
+                         controller=controller)
+        return time.perf_counter() - t
+
+    # calibrate on 32 systems, then size each run for a quarter of `seconds`
+    per_sys = run(32, 1) / 32
+    budget = seconds / 4.0
+    n = int(max(threads * 8, min(NSYS, budget * threads / max(per_sys, 1e-9))))
     n = (n // threads) * threads
-    y = y0[: n * N_SPECIES].copy()
-    t = time.perf_counter()
-    O.gold_integrate(O.SIE, net, 0.0, 1.0e6, 1, nH[:n].copy(), y, n, tol, tol, const_stride=1, threads=threads,
-                     controller=controller)
-    dt = time.perf_counter() - t
+    dt = float(np.median([run(n, threads) for _ in range(3)]))
+    n1 = int(max(8, min(NSYS, budget / max(per_sys, 1e-9))))
+    dt1 = run(n1, 1)
     return {"value": n / dt, "unit": "systems/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} of the {NSYS} systems, {dt:.1f} s, one contiguous slice per thread",
+            "sample": f"first {n} of the {NSYS} systems, one contiguous slice per thread; median of 3 runs, "
+                      f"{dt:.1f} s each",
+            "single_core": {"value": n1 / dt1, "unit": "systems/s", "cores": 1,
+                            "sample": f"first {n1} of the {NSYS} systems, {dt1:.1f} s"},
             "algorithm": ("reference controller" if controller == 0 else "adaptive controller") + f", tol {tol:g}"}
 
 
