@@ -366,6 +366,7 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--dense-nsys", type=int, default=4096)
+    ap.add_argument("--no-dense", action="store_true", help="skip the N = 157 dense roofline and precision forms")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device three times (HBM counter calibration)")
     a = ap.parse_args()
@@ -491,8 +492,8 @@ def main():
 
     mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain,
                                                 not a.no_generated)
-    dense = None if a.no_mixed else dense_roofline(torch, api, W, a.dense_nsys)
-    precision = None if a.no_mixed else dense_precision(torch, api, W, 1024)
+    dense = None if a.no_dense else dense_roofline(torch, api, W, a.dense_nsys)
+    precision = None if a.no_dense else dense_precision(torch, api, W, 1024)
     serve = None if a.no_serve else serving(api, W, a.serve_seconds, a.serve_load)
     cpu = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, REF_TOL, 0)
     cpu_same = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, tol, ctrl)
