@@ -8,8 +8,10 @@ conditions already resident in HBM.  Each step restores the initial state
 (a device-to-device copy) and integrates every system to t_end.
 
 The headline runs the persistent lane-per-system kernel in fast arithmetic
-with the adaptive (error-norm) step controller at abs = rel = 1e-5, the
-setting at which the sampled final states meet rel err <= 1e-3 vs FP64.
+with the adaptive (error-norm) step controller at abs = rel = 2e-6, the
+loosest setting of a sweep at which every sampled final state meets rel err
+<= 1e-3 vs FP64; looser tolerances (faster, with part of the sample outside
+the bound) are reported beside it.
 Also reported (one JSON line): the final-state relative error of a
 512-system sample against FP64 Radau solutions (tests/golden/truth_hhe6.npz),
 per-step latency p50/p99, the integration kernel's roofline position, the
@@ -41,7 +43,10 @@ sys.path.insert(0, ROOT)
 N_SPECIES = 6
 NSYS = 1 << 20
 SEED = 1
-TOL = 1e-5          # headline tolerance (adaptive controller)
+# Headline tolerance (adaptive controller): the loosest of a sweep at which
+# every system of the FP64 sample ends within 1e-3 (at 1e-5, 97% do).
+TOL = 2e-6
+SWEEP_TOLS = (1e-5, 5e-6)  # looser tolerances reported beside the headline
 REF_TOL = 1e-3      # the reference controller's run
 # Algorithmic work per sub-step of hhe6 (counted once, LAPACK-style):
 #   network RHS + Jacobian 674 flops (+ 8 exp, 1 log, 3 div, counted apart),
@@ -71,7 +76,7 @@ def accuracy(yf, nsys):
     e = np.where(np.isnan(e), np.inf, e)
     return {"sample": int(keep.sum()), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
             "median_rel_err": float(np.median(e)), "p90_rel_err": float(np.quantile(e, 0.9)),
-            "nonfinite": int(np.sum(~np.isfinite(e)))}
+            "max_rel_err": float(np.max(e)), "nonfinite": int(np.sum(~np.isfinite(e)))}
 
 
 def cpu_baseline(params, y0, nH, seconds, tol, controller):
@@ -289,7 +294,7 @@ def serving(api, W, seconds, load_frac, seed=5):
             n = int(np.exp(rng.uniform(np.log(1e3), np.log(256e3))))
             y0, c = W.hhe6_systems(n, seed=200 + i)
             blocks.append(dict(kind="hhe6", n=n, N=6, h=srv.block(nets["hhe6"], c, y0, n), out=np.empty_like(y0),
-                               dt=lambda: float(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))), tol=(1e-5, 1e-5)))
+                               dt=lambda: float(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))), tol=(TOL, TOL)))
         else:
             N = int(rng.choice(np.array(W.MIXED_SIZES), p=np.array(W.MIXED_WEIGHTS)))
             net = nets[N][0]
@@ -451,6 +456,25 @@ def main():
         if tr:
             roofline["traffic"] = tr["bytes_per_launch"] * nsys / tr["nsys"]
 
+    # The same kernel at looser tolerances: faster, with part of the systems
+    # outside the 1e-3 bound.
+    sweep = {}
+    if mode == api.MODE_FAST and ctrl == api.CTRL_ADAPTIVE and a.tol is None:
+        for stol in SWEEP_TOLS:
+            ys = torch.empty_like(y0)
+            tms = []
+            for _ in range(2):
+                ys.copy_(y0)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                api.integrate(net, 0.0, W.HHE6_TEND_YR, 1, cst, ys, nsys, stol, stol, method=api.SIE, mode=mode,
+                              kernel=api.KERNEL_LANE, device_pointers=True, stream=stream.cuda_stream,
+                              controller=ctrl)
+                torch.cuda.synchronize()
+                tms.append(time.perf_counter() - t)
+            sweep[f"{stol:g}"] = {"value": nsys / min(tms), "unit": "systems/s",
+                                  "accuracy": accuracy(ys.cpu().numpy().reshape(-1, N_SPECIES), nsys)}
+
     # The reference algorithm as the reference runs it: exact arithmetic,
     # reference controller, tol 1e-3 (one timed pass, same systems).
     y.copy_(y0)
@@ -510,7 +534,7 @@ def main():
                        "request": "one step (all systems)"},
         "substeps_per_system": {"mean": float(sub.mean()), "max": int(sub.max())},
         "status": {"ok": int(np.sum(st == 0)), "stalled": int(np.sum(st == 1)), "exhausted": int(np.sum(st == 2))},
-        "accuracy": acc, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
+        "accuracy": acc, "looser_tolerances": sweep, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
         "reference_algorithm": ref_alg, "plain_gpu": plain,
         "cpu_baseline": cpu, "cpu_same_algorithm": cpu_same,
         "mixed_batch": mixed, "dense_157": dense, "dense_157_precision": precision, "serving": serve,

@@ -182,23 +182,26 @@ def test_mass_action_adaptive_exact(N):
     check(g, r, f"mass_action adaptive N{N}")
 
 
-def test_fast_mode_accuracy_vs_fp64():
-    """Fast arithmetic + adaptive controller at tol 1e-5 on the 512 sampled
-    systems of the 2^20-system hhe6 workload against FP64 Radau solutions:
-    final-state relative error (floor 1e-3) <= 1e-3 for at least 95% of them
-    (the oracle's float32 run of the same controller reaches 97.1%)."""
+@pytest.mark.parametrize("tol,min_frac", [(1e-5, 0.95), (2e-6, 1.0)])
+def test_fast_mode_accuracy_vs_fp64(tol, min_frac):
+    """Fast arithmetic + adaptive controller on the 512 sampled systems of the
+    2^20-system hhe6 workload against FP64 Radau solutions: the fraction with
+    final-state relative error (floor 1e-3) <= 1e-3.  At tol 1e-5 at least 95%
+    (the oracle's float32 run of the same controller reaches 97.1%); at 2e-6,
+    the benchmark's headline setting, every one (oracle: all 512, largest
+    error 8.5e-4)."""
     T = np.load(os.path.join(ROOT, "tests", "golden", "truth_hhe6.npz"))
     idx, yt = T["idx"], T["y"]
     p = W.hhe6_params()
     y0, nH = W.hhe6_systems(int(T["nsys"]), seed=int(T["seed"]))
     Y = y0.reshape(-1, 6)[idx].reshape(-1).copy()
     c = nH[idx].copy()
-    g = run_gpu(api.Network.builtin(api.NET_HHE6, p), 0.0, float(T["tend"]), 1, c, Y, len(idx), 1e-5, api.SIE,
+    g = run_gpu(api.Network.builtin(api.NET_HHE6, p), 0.0, float(T["tend"]), 1, c, Y, len(idx), tol, api.SIE,
                 api.KERNEL_LANE, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE, max_attempts=100000)
     e = np.max(np.abs(g[0].reshape(-1, 6) - yt) / np.maximum(np.abs(yt), 1e-3), axis=1)
     e = np.where(np.isnan(e), np.inf, e)
-    print("fast adaptive: frac<=1e-3", np.mean(e <= 1e-3), "median", np.median(e), "max", e.max())
-    assert np.mean(e <= 1e-3) >= 0.95
+    print(f"fast adaptive tol {tol}: frac<=1e-3", np.mean(e <= 1e-3), "median", np.median(e), "max", e.max())
+    assert np.mean(e <= 1e-3) >= min_frac
     assert (g[2] == 0).all()
 
 
