@@ -448,6 +448,9 @@ def main():
         "kernel": ("lane_fast_kernel<NetHHe6, SIE>" if mode == api.MODE_FAST else "lane_integrate<NetHHe6, SIE, MathExact>"), "kernel_ms": kernel_s * 1e3,
         "valu": {"achieved": algo_flops / kernel_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                  "frac": algo_flops / kernel_s / 1e12 / PEAK_FP32_TFLOPS, "flops_per_substep": F_SUBSTEP},
+        # the roofline that binds this kernel: FP32 vector ALU (no MFMA work;
+        # the HBM figures above are the state in and out, small by design)
+        "binding": "valu",
     }
     trp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(trp):
