@@ -231,6 +231,21 @@ struct NetHHe6 {
         }
     }
 
+    // The same derivative regrouped for fast mode (fewer operations, other
+    // rounding): (g dT_k xe + R dxe_k) y_s = (g xe y_s) dT_k + (R y_s) dxe_k.
+    __device__ static inline void dw_fast(const State &s, float Rv, float g, float G, int sidx, const float *y,
+                                          float *out) {
+        const float dxe[6] = {0.0f, 1.0f, 0.0f, 1.0f, 2.0f, 0.0f};
+        const float ys = y[sidx];
+        const float a = (g * s.xe) * ys, b = Rv * ys;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            float v = fmaf(a, s.dT[k], b * dxe[k]);
+            if (k == sidx) v = v + fmaf(Rv, s.xe, G);
+            out[k] = v;
+        }
+    }
+
     template <class M>
     __device__ static inline void jac(const Params &p, const float *c, const float *y, float *J) {
         State s;
@@ -239,15 +254,32 @@ struct NetHHe6 {
         float Rr = s.Rj[4] + s.Rj[5];
         float gr = s.gj[4] + s.gj[5];
         float dw0[6], dw1[6], dw2[6], dw3[6], dw4[6], dw5[6], dc0[6], dc1[6], dc2[6];
-        dw(s, s.Rj[0], s.gj[0], s.G[0], 0, y, dw0);
-        dw(s, s.Rj[1], s.gj[1], s.G[1], 2, y, dw1);
-        dw(s, s.Rj[2], s.gj[2], s.G[2], 3, y, dw2);
-        dw(s, s.Rj[3], s.gj[3], 0.0f, 1, y, dw3);
-        dw(s, Rr, gr, 0.0f, 3, y, dw4);
-        dw(s, s.Rj[6], s.gj[6], 0.0f, 4, y, dw5);
-        dw(s, s.Rj[0], s.gj[0], 0.0f, 0, y, dc0);
-        dw(s, s.Rj[1], s.gj[1], 0.0f, 2, y, dc1);
-        dw(s, s.Rj[2], s.gj[2], 0.0f, 3, y, dc2);
+        if constexpr (M::kFast) {
+            // dw_j differs from dc_j (the collisional part) only by G_j on its
+            // own species' entry
+            dw_fast(s, s.Rj[0], s.gj[0], 0.0f, 0, y, dc0);
+            dw_fast(s, s.Rj[1], s.gj[1], 0.0f, 2, y, dc1);
+            dw_fast(s, s.Rj[2], s.gj[2], 0.0f, 3, y, dc2);
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                dw0[k] = k == 0 ? dc0[k] + s.G[0] : dc0[k];
+                dw1[k] = k == 2 ? dc1[k] + s.G[1] : dc1[k];
+                dw2[k] = k == 3 ? dc2[k] + s.G[2] : dc2[k];
+            }
+            dw_fast(s, s.Rj[3], s.gj[3], 0.0f, 1, y, dw3);
+            dw_fast(s, Rr, gr, 0.0f, 3, y, dw4);
+            dw_fast(s, s.Rj[6], s.gj[6], 0.0f, 4, y, dw5);
+        } else {
+            dw(s, s.Rj[0], s.gj[0], s.G[0], 0, y, dw0);
+            dw(s, s.Rj[1], s.gj[1], s.G[1], 2, y, dw1);
+            dw(s, s.Rj[2], s.gj[2], s.G[2], 3, y, dw2);
+            dw(s, s.Rj[3], s.gj[3], 0.0f, 1, y, dw3);
+            dw(s, Rr, gr, 0.0f, 3, y, dw4);
+            dw(s, s.Rj[6], s.gj[6], 0.0f, 4, y, dw5);
+            dw(s, s.Rj[0], s.gj[0], 0.0f, 0, y, dc0);
+            dw(s, s.Rj[1], s.gj[1], 0.0f, 2, y, dc1);
+            dw(s, s.Rj[2], s.gj[2], 0.0f, 3, y, dc2);
+        }
         float w3 = (s.Rj[3] * xe) * y[1];
         float w4 = (Rr * xe) * y[3];
         float w5 = (s.Rj[6] * xe) * y[4];
