@@ -207,6 +207,8 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
     if (a.controller == WIND_CTRL_REFERENCE) {
 #pragma unroll
         for (int i = 0; i < N; i++) flag = flag || species_fails<METHOD>(y1[i], y2[i], a.ABS, a.REL);
+    } else if constexpr (M::kFast) {
+        err = err_norm_fast<N>(y1, y2, a.ABS, a.REL);
     } else {
 #pragma unroll
         for (int i = 0; i < N; i++) err = err_max(err, err_term(y1[i], y2[i], a.ABS, a.REL));

@@ -294,6 +294,7 @@ struct NetHHe6 {
         float cci = (((s.Rj[0] * xe) * y[0]) * CHI0 + ((s.Rj[1] * xe) * y[2]) * CHI1) +
                     ((s.Rj[2] * xe) * y[3]) * CHI2;
         float cool = (cci + W * recT) + LNs * xion;
+        const float WRT = W * (RECC * 1.0e-4f), LFT = (LN * dsqc) * xion;  // fast mode only
         const float dxe[6] = {0.0f, 1.0f, 0.0f, 1.0f, 2.0f, 0.0f};
         const float dion[6] = {0.0f, 1.0f, 0.0f, 1.0f, 4.0f, 0.0f};
 #pragma unroll
@@ -309,8 +310,15 @@ struct NetHHe6 {
             if (k == 2) heat = s.G[1] * E1;
             if (k == 3) heat = s.G[2] * E2;
             float dcci = (dc0[k] * CHI0 + dc1[k] * CHI1) + dc2[k] * CHI2;
-            float dcrec = ((dw3[k] + dw4[k]) + dw5[k]) * recT + W * (RECC * (s.dT[k] * 1.0e-4f));
-            float dcff = (LN * (dsqc * s.dT[k])) * xion + LNs * (dxe[k] * ion + xe * dion[k]);
+            float dcrec, dcff;
+            if constexpr (M::kFast) {
+                // the dT_k factors of both terms gathered per k (other rounding)
+                dcrec = fmaf((dw3[k] + dw4[k]) + dw5[k], recT, WRT * s.dT[k]);
+                dcff = fmaf(LFT, s.dT[k], LNs * (dxe[k] * ion + xe * dion[k]));
+            } else {
+                dcrec = ((dw3[k] + dw4[k]) + dw5[k]) * recT + W * (RECC * (s.dT[k] * 1.0e-4f));
+                dcff = (LN * (dsqc * s.dT[k])) * xion + LNs * (dxe[k] * ion + xe * dion[k]);
+            }
             col[5] = heat - (((dcci + dcrec) + dcff) * s.phi + cool * s.dphi[k]);
         }
     }
@@ -387,5 +395,26 @@ __device__ inline double err_term(float y1, float y2, float ABS, float REL) {
 
 // max() that keeps a NaN once one has been seen (a NaN norm must fail).
 __device__ inline double err_max(double a, double b) { return (b > a || b != b || a != a) ? (a != a ? a : b) : a; }
+
+// The adaptive norm for fast mode: the same terms in double, the largest
+// selected by cross-multiplication and divided once (the result may differ
+// from the largest separately rounded quotient by its last bit).  Any NaN
+// term, including 0/0, makes the norm NaN as err_term and err_max do.
+template <int N>
+__device__ inline double err_norm_fast(const float (&y1)[N], const float (&y2)[N], float ABS, float REL) {
+    double nb = 0.0, db = 1.0;
+    bool nan = false;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double num = fabs((double)y2[i] - (double)y1[i]);
+        const double den = (double)ABS + (double)REL * fabs((double)y2[i]);
+        nan = nan || num != num || den != den || (num == 0.0 && den == 0.0);
+        if (num * db > nb * den) {
+            nb = num;
+            db = den;
+        }
+    }
+    return nan ? __builtin_nan("") : nb / db;
+}
 
 }  // namespace wind
