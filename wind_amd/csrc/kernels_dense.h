@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
                 for (int wv = 1; wv < (T + 63) / 64; wv++) err = err_max(err, s_err[wv]);
             }
             attempts++;
-            const bool acc = control_step<METHOD>(a.controller, a.tend, hh, flag, err, tnow, h, unsolved, status);
+            const bool acc = control_step<METHOD, true>(a.controller, a.tend, hh, flag, err, tnow, h, unsolved, status);
             if (acc) {
                 const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
                 for (int i = t; i < N; i += T) L.y[i] = extrap ? 2.0f * L.yw[i] - L.y1[i] : L.yw[i];

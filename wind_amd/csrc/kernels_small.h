@@ -215,7 +215,8 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
     }
     s.attempts++;
     const float hh_used = hh;
-    if (control_step<METHOD>(a.controller, a.tend, hh_used, flag, err, s.tnow, s.h, s.unsolved, s.status)) {
+    if (control_step<METHOD, M::kFast>(a.controller, a.tend, hh_used, flag, err, s.tnow, s.h, s.unsolved,
+                                       s.status)) {
         const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
 #pragma unroll
         for (int i = 0; i < N; i++) s.y[i] = extrap ? 2.0f * y2[i] - y1[i] : y2[i];

@@ -359,7 +359,10 @@ __device__ inline bool species_fails(float y1, float y2, float ABS, float REL) {
 // ---------------------------------------------------------------------------
 constexpr int kRefineCap = 10;  // common_gold.c:121
 
-template <int METHOD>
+// FAST (fast-mode kernels): the step factor in float with a hardware
+// reciprocal square root instead of the double sqrt and division; the
+// accept test is the same.
+template <int METHOD, bool FAST = false>
 __device__ inline bool control_step(int controller, float tend, float hh, bool ref_flag, double err,
                                     float &tnow, float &h, int &unsolved, int &status) {
     if (controller == WIND_CTRL_REFERENCE) {
@@ -376,13 +379,26 @@ __device__ inline bool control_step(int controller, float tend, float hh, bool r
         return true;
     }
     const bool ok = err <= 1.0;
-    double fac;
-    if (ok)
-        fac = err > 0.0 ? 0.9 / sqrt(err) : 4.0;
-    else
-        fac = (err == err) ? 0.9 / sqrt(err) : 0.2;
-    fac = fmin(4.0, fmax(0.2, fac));
-    h = (float)((double)hh * fac);
+    if constexpr (FAST) {
+        // err beyond float range gives rsq(inf) = 0 or rsq(0) = inf; the clamp
+        // takes both to the ends of [0.2, 4]
+        const float r = 0.9f * __builtin_amdgcn_rsqf((float)err);
+        float fac;
+        if (ok)
+            fac = err > 0.0 ? r : 4.0f;
+        else
+            fac = (err == err) ? r : 0.2f;
+        fac = fminf(4.0f, fmaxf(0.2f, fac));
+        h = hh * fac;
+    } else {
+        double fac;
+        if (ok)
+            fac = err > 0.0 ? 0.9 / sqrt(err) : 4.0;
+        else
+            fac = (err == err) ? 0.9 / sqrt(err) : 0.2;
+        fac = fmin(4.0, fmax(0.2, fac));
+        h = (float)((double)hh * fac);
+    }
     if (ok) tnow = tnow + hh;
     if (h == 0.0f && tnow < tend) status = WIND_STATUS_STALLED;
     return ok;
