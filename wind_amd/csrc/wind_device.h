@@ -57,13 +57,18 @@ struct MathExact {
         return (float)e * 0.693147181f + (t + t);
     }
     __device__ static inline float mad(float a, float b, float c) { return a * b + c; }
+    __device__ static inline float div(float a, float b) { return a / b; }
 };
 
 struct MathFast {
     static constexpr bool kFast = true;
     __device__ static inline float ex(float x) { return __expf(x); }
-    __device__ static inline float lg(float x) { return __logf(x); }
+    // log2 on the transcendental unit times ln 2; x must be a normal float
+    // (the only caller takes the log of a temperature of at least 10 K)
+    __device__ static inline float lg(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
     __device__ static inline float mad(float a, float b, float c) { return fmaf(a, b, c); }
+    // hardware reciprocal (1 ulp) instead of the IEEE division sequence
+    __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 };
 
 // ---------------------------------------------------------------------------
@@ -159,14 +164,14 @@ struct NetHHe6 {
                                         bool want_derivs) {
         s.xe = (y[1] + y[3]) + (y[4] + y[4]);
         s.mu = (((y[0] + y[1]) + (y[2] + y[3])) + y[4]) + s.xe;
-        float dTde = TUNIT / (1.5f * s.mu);
+        float dTde = M::div(TUNIT, 1.5f * s.mu);
         float Traw = y[5] * dTde;
         bool clamped = !(Traw > TFLOOR);
         float T = clamped ? TFLOOR : Traw;
         s.phi = clamped ? Traw * 0.1f : 1.0f;
         s.T = T;
         s.lnT = M::lg(T);
-        s.invT = 1.0f / T;
+        s.invT = M::div(1.0f, T);
         s.nHyr = c[0] * YR;
 #pragma unroll
         for (int j = 0; j < 7; j++) {
@@ -177,7 +182,7 @@ struct NetHHe6 {
         for (int j = 0; j < 3; j++) s.G[j] = p.Gamma[j] * YR;
         s.sqT = M::ex(0.5f * s.lnT);
         if (want_derivs) {
-            float Tmu = Traw / s.mu;
+            float Tmu = M::div(Traw, s.mu);
             const float dmu[5] = {1.0f, 2.0f, 1.0f, 2.0f, 3.0f};
             float dTraw[6];
 #pragma unroll
