@@ -170,17 +170,30 @@ struct NetHHe6 {
         float T = clamped ? TFLOOR : Traw;
         s.phi = clamped ? Traw * 0.1f : 1.0f;
         s.T = T;
-        s.lnT = M::lg(T);
         s.invT = M::div(1.0f, T);
         s.nHyr = c[0] * YR;
+        if constexpr (M::kFast) {
+            // in base 2 on the transcendental unit: A T^b e^(-C/T) =
+            // 2^(log2(e) ln A + b log2 T - log2(e) C / T); T >= 10 K is a normal float
+            constexpr float LOG2E = 1.44269504f;
+            const float l2T = __builtin_amdgcn_logf(T);
+            s.lnT = l2T * 0.693147181f;
 #pragma unroll
-        for (int j = 0; j < 7; j++) {
-            s.Rj[j] = M::ex((p.lnA[j] + p.b[j] * s.lnT) - p.C[j] * s.invT) * s.nHyr;
-            if (want_derivs) s.gj[j] = s.Rj[j] * ((p.b[j] + p.C[j] * s.invT) * s.invT);
+            for (int j = 0; j < 7; j++)
+                s.Rj[j] = __builtin_amdgcn_exp2f(fmaf(p.b[j], l2T, p.lnA[j] * LOG2E) - (p.C[j] * LOG2E) * s.invT) *
+                          s.nHyr;
+            s.sqT = __builtin_amdgcn_sqrtf(T);
+        } else {
+            s.lnT = M::lg(T);
+#pragma unroll
+            for (int j = 0; j < 7; j++) s.Rj[j] = M::ex((p.lnA[j] + p.b[j] * s.lnT) - p.C[j] * s.invT) * s.nHyr;
+            s.sqT = M::ex(0.5f * s.lnT);
         }
+        if (want_derivs)
+#pragma unroll
+            for (int j = 0; j < 7; j++) s.gj[j] = s.Rj[j] * ((p.b[j] + p.C[j] * s.invT) * s.invT);
 #pragma unroll
         for (int j = 0; j < 3; j++) s.G[j] = p.Gamma[j] * YR;
-        s.sqT = M::ex(0.5f * s.lnT);
         if (want_derivs) {
             float Tmu = M::div(Traw, s.mu);
             const float dmu[5] = {1.0f, 2.0f, 1.0f, 2.0f, 3.0f};
@@ -236,18 +249,63 @@ struct NetHHe6 {
         }
     }
 
-    // The same derivative regrouped for fast mode (fewer operations, other
-    // rounding): (g dT_k xe + R dxe_k) y_s = (g xe y_s) dT_k + (R y_s) dxe_k.
-    __device__ static inline void dw_fast(const State &s, float Rv, float g, float G, int sidx, const float *y,
-                                          float *out) {
+    // Fast-mode Jacobian from its structure (other rounding than jac's exact
+    // form): every term w_j = (R_j xe + G_j) y_(s_j) has the derivative
+    // a_j dT_k + b_j dxe_k + [k == s_j] (R_j xe + G_j) with a_j = g_j xe y_(s_j)
+    // and b_j = R_j y_(s_j), so the species rows are U_i dT_k + V_i dxe_k plus
+    // a few entries on their own species, and the energy row is
+    // heat_k - ((ET dT_k + EX dxe_k + EI dion_k + s_k) phi + cool dphi_k).
+    __device__ static inline void jac_fast(const State &s, const float *y, float *J) {
+        const float xe = s.xe;
+        const float Rr = s.Rj[4] + s.Rj[5], gr = s.gj[4] + s.gj[5];
+        // w0..w5 of rhs: species s_j = 0, 2, 3, 1, 3, 4
+        const float a0 = (s.gj[0] * xe) * y[0], b0 = s.Rj[0] * y[0], e0 = s.Rj[0] * xe;
+        const float a1 = (s.gj[1] * xe) * y[2], b1 = s.Rj[1] * y[2], e1 = s.Rj[1] * xe;
+        const float a2 = (s.gj[2] * xe) * y[3], b2 = s.Rj[2] * y[3], e2 = s.Rj[2] * xe;
+        const float a3 = (s.gj[3] * xe) * y[1], b3 = s.Rj[3] * y[1], e3 = s.Rj[3] * xe;
+        const float a4 = (gr * xe) * y[3], b4 = Rr * y[3], e4 = Rr * xe;
+        const float a5 = (s.gj[6] * xe) * y[4], b5 = s.Rj[6] * y[4], e5 = s.Rj[6] * xe;
         const float dxe[6] = {0.0f, 1.0f, 0.0f, 1.0f, 2.0f, 0.0f};
-        const float ys = y[sidx];
-        const float a = (g * s.xe) * ys, b = Rv * ys;
+        const float dion[6] = {0.0f, 1.0f, 0.0f, 1.0f, 4.0f, 0.0f};
+        // species rows (f0 = w3 - w0, f1 = -f0, f2 = w4 - w1, f3 = w1 + w5 - w2 - w4, f4 = w2 - w5)
+        const float U[5] = {a3 - a0, a0 - a3, a4 - a1, (a1 + a5) - (a2 + a4), a2 - a5};
+        const float V[5] = {b3 - b0, b0 - b3, b4 - b1, (b1 + b5) - (b2 + b4), b2 - b5};
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+#pragma unroll
+            for (int i = 0; i < 5; i++) J[6 * k + i] = fmaf(U[i], s.dT[k], V[i] * dxe[k]);
+        const float f0 = e0 + s.G[0], f1 = e1 + s.G[1], f2 = e2 + s.G[2];
+        J[0] -= f0;           // d f0 / d y0
+        J[1] += f0;           // d f1 / d y0
+        J[6] += e3;           // d f0 / d y1
+        J[7] -= e3;           // d f1 / d y1
+        J[14] -= f1;          // d f2 / d y2
+        J[15] += f1;          // d f3 / d y2
+        J[20] += e4;          // d f2 / d y3
+        J[21] -= f2 + e4;     // d f3 / d y3
+        J[22] += f2;          // d f4 / d y3
+        J[27] += e5;          // d f3 / d y4
+        J[28] -= e5;          // d f4 / d y4
+        // energy row
+        const float W = ((e3 * y[1]) + (e4 * y[3])) + (e5 * y[4]);
+        const float recT = RECC * (s.T * 1.0e-4f);
+        const float ion = (y[1] + y[3]) + 4.0f * y[4];
+        const float LN = LFF * s.nHyr;
+        const float LNs = LN * s.sqT;
+        const float xion = xe * ion;
+        const float dsqc = (0.5f * s.sqT) * s.invT;
+        const float cci = ((e0 * y[0]) * CHI0 + (e1 * y[2]) * CHI1) + (e2 * y[3]) * CHI2;
+        const float cool = (cci + W * recT) + LNs * xion;
+        const float ET = ((CHI0 * a0 + CHI1 * a1) + CHI2 * a2) + ((a3 + a4) + a5) * recT +
+                         (W * (RECC * 1.0e-4f) + (LN * dsqc) * xion);
+        const float EX = ((CHI0 * b0 + CHI1 * b1) + CHI2 * b2) + ((b3 + b4) + b5) * recT + LNs * ion;
+        const float EI = LNs * xe;
+        const float sp[6] = {CHI0 * e0, recT * e3, CHI1 * e1, CHI2 * e2 + recT * e4, recT * e5, 0.0f};
+        const float heat[6] = {s.G[0] * E0, 0.0f, s.G[1] * E1, s.G[2] * E2, 0.0f, 0.0f};
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-            float v = fmaf(a, s.dT[k], b * dxe[k]);
-            if (k == sidx) v = v + fmaf(Rv, s.xe, G);
-            out[k] = v;
+            const float sum = fmaf(ET, s.dT[k], EX * dxe[k] + EI * dion[k]) + sp[k];
+            J[6 * k + 5] = heat[k] - fmaf(sum, s.phi, cool * s.dphi[k]);
         }
     }
 
@@ -255,36 +313,23 @@ struct NetHHe6 {
     __device__ static inline void jac(const Params &p, const float *c, const float *y, float *J) {
         State s;
         state<M>(p, c, y, s, true);
+        if constexpr (M::kFast) {
+            jac_fast(s, y, J);
+            return;
+        }
         const float xe = s.xe;
         float Rr = s.Rj[4] + s.Rj[5];
         float gr = s.gj[4] + s.gj[5];
         float dw0[6], dw1[6], dw2[6], dw3[6], dw4[6], dw5[6], dc0[6], dc1[6], dc2[6];
-        if constexpr (M::kFast) {
-            // dw_j differs from dc_j (the collisional part) only by G_j on its
-            // own species' entry
-            dw_fast(s, s.Rj[0], s.gj[0], 0.0f, 0, y, dc0);
-            dw_fast(s, s.Rj[1], s.gj[1], 0.0f, 2, y, dc1);
-            dw_fast(s, s.Rj[2], s.gj[2], 0.0f, 3, y, dc2);
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                dw0[k] = k == 0 ? dc0[k] + s.G[0] : dc0[k];
-                dw1[k] = k == 2 ? dc1[k] + s.G[1] : dc1[k];
-                dw2[k] = k == 3 ? dc2[k] + s.G[2] : dc2[k];
-            }
-            dw_fast(s, s.Rj[3], s.gj[3], 0.0f, 1, y, dw3);
-            dw_fast(s, Rr, gr, 0.0f, 3, y, dw4);
-            dw_fast(s, s.Rj[6], s.gj[6], 0.0f, 4, y, dw5);
-        } else {
-            dw(s, s.Rj[0], s.gj[0], s.G[0], 0, y, dw0);
-            dw(s, s.Rj[1], s.gj[1], s.G[1], 2, y, dw1);
-            dw(s, s.Rj[2], s.gj[2], s.G[2], 3, y, dw2);
-            dw(s, s.Rj[3], s.gj[3], 0.0f, 1, y, dw3);
-            dw(s, Rr, gr, 0.0f, 3, y, dw4);
-            dw(s, s.Rj[6], s.gj[6], 0.0f, 4, y, dw5);
-            dw(s, s.Rj[0], s.gj[0], 0.0f, 0, y, dc0);
-            dw(s, s.Rj[1], s.gj[1], 0.0f, 2, y, dc1);
-            dw(s, s.Rj[2], s.gj[2], 0.0f, 3, y, dc2);
-        }
+        dw(s, s.Rj[0], s.gj[0], s.G[0], 0, y, dw0);
+        dw(s, s.Rj[1], s.gj[1], s.G[1], 2, y, dw1);
+        dw(s, s.Rj[2], s.gj[2], s.G[2], 3, y, dw2);
+        dw(s, s.Rj[3], s.gj[3], 0.0f, 1, y, dw3);
+        dw(s, Rr, gr, 0.0f, 3, y, dw4);
+        dw(s, s.Rj[6], s.gj[6], 0.0f, 4, y, dw5);
+        dw(s, s.Rj[0], s.gj[0], 0.0f, 0, y, dc0);
+        dw(s, s.Rj[1], s.gj[1], 0.0f, 2, y, dc1);
+        dw(s, s.Rj[2], s.gj[2], 0.0f, 3, y, dc2);
         float w3 = (s.Rj[3] * xe) * y[1];
         float w4 = (Rr * xe) * y[3];
         float w5 = (s.Rj[6] * xe) * y[4];
@@ -299,7 +344,6 @@ struct NetHHe6 {
         float cci = (((s.Rj[0] * xe) * y[0]) * CHI0 + ((s.Rj[1] * xe) * y[2]) * CHI1) +
                     ((s.Rj[2] * xe) * y[3]) * CHI2;
         float cool = (cci + W * recT) + LNs * xion;
-        const float WRT = W * (RECC * 1.0e-4f), LFT = (LN * dsqc) * xion;  // fast mode only
         const float dxe[6] = {0.0f, 1.0f, 0.0f, 1.0f, 2.0f, 0.0f};
         const float dion[6] = {0.0f, 1.0f, 0.0f, 1.0f, 4.0f, 0.0f};
 #pragma unroll
@@ -315,15 +359,8 @@ struct NetHHe6 {
             if (k == 2) heat = s.G[1] * E1;
             if (k == 3) heat = s.G[2] * E2;
             float dcci = (dc0[k] * CHI0 + dc1[k] * CHI1) + dc2[k] * CHI2;
-            float dcrec, dcff;
-            if constexpr (M::kFast) {
-                // the dT_k factors of both terms gathered per k (other rounding)
-                dcrec = fmaf((dw3[k] + dw4[k]) + dw5[k], recT, WRT * s.dT[k]);
-                dcff = fmaf(LFT, s.dT[k], LNs * (dxe[k] * ion + xe * dion[k]));
-            } else {
-                dcrec = ((dw3[k] + dw4[k]) + dw5[k]) * recT + W * (RECC * (s.dT[k] * 1.0e-4f));
-                dcff = (LN * (dsqc * s.dT[k])) * xion + LNs * (dxe[k] * ion + xe * dion[k]);
-            }
+            float dcrec = ((dw3[k] + dw4[k]) + dw5[k]) * recT + W * (RECC * (s.dT[k] * 1.0e-4f));
+            float dcff = (LN * (dsqc * s.dT[k])) * xion + LNs * (dxe[k] * ion + xe * dion[k]);
             col[5] = heat - (((dcci + dcrec) + dcff) * s.phi + cool * s.dphi[k]);
         }
     }
