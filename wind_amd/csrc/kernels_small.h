@@ -102,6 +102,24 @@ __device__ inline void lu_solve_fast(float (&Ms)[N * N], const float (&f)[N], fl
     }
 }
 
+// Fast-mode SIE update y + h x with (I - h J) x = f, solved as
+// (J - I/h) z = -f, z = h x: the shift touches only the diagonal (no h J
+// products) and the update is y + z.  A step that rounds to zero (h = 0)
+// leaves y as it is.  The factorisation overwrites J.
+template <int N>
+__device__ inline void sie_update_fast(float (&J)[N * N], const float (&f)[N], float hs, const float (&y)[N],
+                                       float (&out)[N]) {
+    const float c = __builtin_amdgcn_rcpf(hs);
+#pragma unroll
+    for (int i = 0; i < N; i++) J[i * (N + 1)] = J[i * (N + 1)] - c;
+    float nf[N], z[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) nf[i] = -f[i];
+    lu_solve_fast<N>(J, nf, z);
+#pragma unroll
+    for (int i = 0; i < N; i++) out[i] = hs != 0.0f ? y[i] + z[i] : y[i];
+}
+
 // One sub-step of take_step (sie_gold.c:54-77, rk2_gold.c:27-44) on a
 // register-resident state.
 template <class Net, int METHOD, class M>
@@ -115,18 +133,22 @@ __device__ inline void substep(const typename Net::Params &p, const float *c, fl
 #pragma unroll
         for (int e = 0; e < N * N; e++) A[e] = 0.0f;
         Net::template jac<M>(p, c, y, A);
-        const float nh = -hs;
+        if constexpr (M::kFast) {
+            float yn[N];
+            sie_update_fast<N>(A, f, hs, y, yn);
 #pragma unroll
-        for (int e = 0; e < N * N; e++) A[e] = nh * A[e];
+            for (int i = 0; i < N; i++) y[i] = yn[i];
+        } else {
+            const float nh = -hs;
 #pragma unroll
-        for (int i = 0; i < N; i++) A[i * (N + 1)] = A[i * (N + 1)] + 1.0f;
-        float g[N];
-        if (M::kFast)
-            lu_solve_fast<N>(A, f, g);
-        else
+            for (int e = 0; e < N * N; e++) A[e] = nh * A[e];
+#pragma unroll
+            for (int i = 0; i < N; i++) A[i * (N + 1)] = A[i * (N + 1)] + 1.0f;
+            float g[N];
             gje_solve_exact<N>(A, f, g);
 #pragma unroll
-        for (int i = 0; i < N; i++) y[i] = M::mad(hs, g[i], y[i]);
+            for (int i = 0; i < N; i++) y[i] = M::mad(hs, g[i], y[i]);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < N; i++) y[i] = M::mad(hs, f[i], y[i]);
@@ -148,6 +170,14 @@ __device__ inline void substep_pair(const typename Net::Params &p, const float *
 #pragma unroll
         for (int e = 0; e < N * N; e++) J[e] = 0.0f;
         Net::template jac<M>(p, c, y, J);
+        if constexpr (M::kFast) {
+            float A[N * N];
+#pragma unroll
+            for (int e = 0; e < N * N; e++) A[e] = J[e];
+            sie_update_fast<N>(A, f, hs1, y, y1);
+            sie_update_fast<N>(J, f, hs2, y, y2);
+            return;
+        }
 #pragma unroll
         for (int pass = 0; pass < 2; pass++) {
             const float hs = pass == 0 ? hs1 : hs2;
@@ -158,10 +188,7 @@ __device__ inline void substep_pair(const typename Net::Params &p, const float *
 #pragma unroll
             for (int i = 0; i < N; i++) A[i * (N + 1)] = A[i * (N + 1)] + 1.0f;
             float g[N];
-            if (M::kFast)
-                lu_solve_fast<N>(A, f, g);
-            else
-                gje_solve_exact<N>(A, f, g);
+            gje_solve_exact<N>(A, f, g);
 #pragma unroll
             for (int i = 0; i < N; i++) {
                 if (pass == 0)

@@ -23,6 +23,11 @@ __global__ void __launch_bounds__(256, WIND_LANE_FAST_MIN_BLOCKS)
 
 }  // namespace
 
+// Persistent blocks per CU.  Fewer lanes than the registers allow run
+// faster: on the 2^20-system hhe6 workload (SIE, tol 2e-6) a launch took
+// 70.3 / 61.6 / 66.3 / 71.8 ms with 1 / 2 / 3 / 4 blocks (waves per SIMD).
+constexpr int kLaneFastBlocksPerCU = 2;
+
 template <class Net, int METHOD>
 int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
@@ -30,6 +35,7 @@ int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateA
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    per_cu = per_cu < kLaneFastBlocksPerCU ? per_cu : kLaneFastBlocksPerCU;
     const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys, 256);
     hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     e = hipGetLastError();
