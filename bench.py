@@ -114,7 +114,9 @@ def cpu_baseline(params, y0, nH, seconds, tol, controller):
 
 # ----------------------------------------------------------- mixed-N batch
 MIXED_NSYS = 1 << 18
-MA_ABS, MA_REL = 1e-10, 1e-4     # mass-action controller tolerances
+# Mass-action controller tolerances: every system of the FP64 sample ends
+# within 1e-3 (at rel 1e-4 the worst N = 5 system ends at 1.2e-3).
+MA_ABS, MA_REL = 1e-10, 8e-5
 PLAIN_SAMPLE = {5: 8192, 10: 8192, 32: 4096, 64: 2048, 157: 256}
 
 

@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TRUTH = os.path.join(ROOT, "tests", "golden", "truth_mass_action.npz")
 # Controller tolerances of the mass-action workloads: a small absolute floor
 # (species span 1e-12..1) so that trace species are controlled relatively.
-MA_ABS, MA_REL = 1e-10, 1e-4
+MA_ABS, MA_REL = 1e-10, 8e-5
 
 
 def torch_cuda():
