@@ -104,8 +104,11 @@ __device__ inline void lu_solve_fast(float (&Ms)[N * N], const float (&f)[N], fl
 
 // Fast-mode SIE update y + h x with (I - h J) x = f, solved as
 // (J - I/h) z = -f, z = h x: the shift touches only the diagonal (no h J
-// products) and the update is y + z.  A step that rounds to zero (h = 0)
-// leaves y as it is.  The factorisation overwrites J.
+// products) and the update is y + z.  A step that rounds to zero, or whose
+// reciprocal overflows, shifts the diagonal to -inf: the pivot reciprocals
+// are -0, every multiplier is +-0 and, for finite J and f, z = +-0 and y
+// stays as it is (a non-finite f or J gives NaN, as y + 0 (I - 0 J)^-1 f
+// does).  The factorisation overwrites J.
 template <int N>
 __device__ inline void sie_update_fast(float (&J)[N * N], const float (&f)[N], float hs, const float (&y)[N],
                                        float (&out)[N]) {
@@ -117,7 +120,7 @@ __device__ inline void sie_update_fast(float (&J)[N * N], const float (&f)[N], f
     for (int i = 0; i < N; i++) nf[i] = -f[i];
     lu_solve_fast<N>(J, nf, z);
 #pragma unroll
-    for (int i = 0; i < N; i++) out[i] = hs != 0.0f ? y[i] + z[i] : y[i];
+    for (int i = 0; i < N; i++) out[i] = y[i] + z[i];
 }
 
 // One sub-step of take_step (sie_gold.c:54-77, rk2_gold.c:27-44) on a

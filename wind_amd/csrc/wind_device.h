@@ -454,14 +454,28 @@ __device__ inline double err_term(float y1, float y2, float ABS, float REL) {
 // max() that keeps a NaN once one has been seen (a NaN norm must fail).
 __device__ inline double err_max(double a, double b) { return (b > a || b != b || a != a) ? (a != a ? a : b) : a; }
 
-// The adaptive norm for fast mode: the same terms in double, the largest
-// selected by cross-multiplication and divided once (the result may differ
-// from the largest separately rounded quotient by its last bit).  Any NaN
-// term, including 0/0, makes the norm NaN as err_term and err_max do.
+// The adaptive norm for fast mode.  With ABS >= 2^-120 every denominator
+// ABS + REL |y2| is a normal float whose hardware reciprocal is finite, so
+// the terms are taken in float as |y2 - y1| * rcp(den) (a few ulp from the
+// double quotient; a term beyond float range is +inf and fails, as its
+// double value would).  A smaller ABS (a uniform kernel argument, so the
+// branch does not diverge) takes the double terms, the largest selected by
+// cross-multiplication and divided once.  Either way any NaN term,
+// including 0/0, makes the norm NaN as err_term and err_max do.
 template <int N>
 __device__ inline double err_norm_fast(const float (&y1)[N], const float (&y2)[N], float ABS, float REL) {
-    double nb = 0.0, db = 1.0;
     bool nan = false;
+    if (ABS >= 0x1p-120f) {
+        float m = 0.0f;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const float r = fabsf(y2[i] - y1[i]) * __builtin_amdgcn_rcpf(fmaf(REL, fabsf(y2[i]), ABS));
+            nan = nan || r != r;
+            m = fmaxf(m, r);
+        }
+        return nan ? __builtin_nan("") : (double)m;
+    }
+    double nb = 0.0, db = 1.0;
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const double num = fabs((double)y2[i] - (double)y1[i]);
