@@ -11,7 +11,10 @@ The headline runs the persistent lane-per-system kernel in fast arithmetic
 with the adaptive (error-norm) step controller at abs = rel = 2e-6, the
 loosest setting of a sweep at which every sampled final state meets rel err
 <= 1e-3 vs FP64; looser tolerances (faster, with part of the sample outside
-the bound) are reported beside it.
+the bound) are reported beside it, and so is the same integration with the
+previous step's per-system sub-step counts as cost hints (longest systems
+first; bit-identical results), the case of a caller that integrates the
+same cells again.
 Also reported (one JSON line): the final-state relative error of a
 512-system sample against FP64 Radau solutions (tests/golden/truth_hhe6.npz),
 per-step latency p50/p99, the integration kernel's roofline position, the
@@ -374,6 +377,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--dense-nsys", type=int, default=4096)
     ap.add_argument("--no-dense", action="store_true", help="skip the N = 157 dense roofline and precision forms")
+    ap.add_argument("--hint-steps", type=int, default=3,
+                    help="timed steps of the headline with the previous step's sub-step counts as cost hints")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device three times (HBM counter calibration)")
     a = ap.parse_args()
@@ -461,6 +466,33 @@ def main():
         if tr:
             roofline["traffic"] = tr["bytes_per_launch"] * nsys / tr["nsys"]
 
+    # Start order from cost hints: each system's sub-step count from the step
+    # before (what a caller integrating the same cells again would pass) as
+    # wind_options.cost_hint.  Same kernel and arithmetic, longest systems
+    # first; the results must be bit-identical to the unhinted step's.
+    hinted = None
+    if mode == api.MODE_FAST and a.hint_steps > 0:
+        hint = steps_d.clone()
+        yh = torch.empty_like(y0)
+        wall, kern = [], []
+        for i in range(1 + a.hint_steps):
+            torch.cuda.synchronize()
+            s = time.perf_counter()
+            yh.copy_(y0)
+            _, kms = api.integrate(net, 0.0, W.HHE6_TEND_YR, 1, cst, yh, nsys, tol, tol, method=api.SIE,
+                                   mode=mode, kernel=api.KERNEL_LANE, device_pointers=True,
+                                   stream=stream.cuda_stream, want_kernel_ms=True, controller=ctrl,
+                                   cost_hint=hint)
+            torch.cuda.synchronize()
+            if i:
+                wall.append(time.perf_counter() - s)
+                kern.append(kms)
+        hinted = {"value": nsys / float(np.mean(wall)), "unit": "systems/s",
+                  "ms_per_step": float(np.mean(wall)) * 1e3, "kernel_ms": float(np.mean(kern)),
+                  "steps": a.hint_steps,
+                  "identical_results": bool(torch.equal(yh.view(torch.int32), y.view(torch.int32))),
+                  "hint": "each system's sub-step count from the previous step (steps_out), passed as cost_hint"}
+
     # The same kernel at looser tolerances: faster, with part of the systems
     # outside the 1e-3 bound.
     sweep = {}
@@ -539,7 +571,7 @@ def main():
                        "request": "one step (all systems)"},
         "substeps_per_system": {"mean": float(sub.mean()), "max": int(sub.max())},
         "status": {"ok": int(np.sum(st == 0)), "stalled": int(np.sum(st == 1)), "exhausted": int(np.sum(st == 2))},
-        "accuracy": acc, "looser_tolerances": sweep, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
+        "accuracy": acc, "with_cost_hint": hinted, "looser_tolerances": sweep, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
         "reference_algorithm": ref_alg, "plain_gpu": plain,
         "cpu_baseline": cpu, "cpu_same_algorithm": cpu_same,
         "mixed_batch": mixed, "dense_157": dense, "dense_157_precision": precision, "serving": serve,

@@ -162,6 +162,14 @@ typedef struct wind_options {
     int solver;            /* wind_solver (fast mode, mass-action networks) */
     int ir_sweeps;         /* iterative-refinement sweeps after the solve (0-4) */
     int flags;             /* WIND_FLAG_* */
+    const int32_t *cost_hint; /* wind_integrate, lane kernels (compiled-in and
+                              generated networks): optional [Nsystems]
+                              per-system cost estimates, device or host as
+                              the other arrays (e.g. the previous call's
+                              steps_out for the same cells).  Systems start in
+                              decreasing order of it, so that long ones do not
+                              finish alone at the end of the run.  Results do
+                              not depend on it. */
 } wind_options;
 
 /* wind_options.flags */

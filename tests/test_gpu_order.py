@@ -1,0 +1,81 @@
+"""GPU: start order from per-system cost hints (wind_options.cost_hint),
+through the C-ABI.
+
+The lane kernels take systems from a queue; a cost hint only permutes that
+queue (longest first), and each system's arithmetic does not depend on the
+lane that runs it.  So final states, per-system step counts, status and
+totals must be bit-identical with and without the hint, in both arithmetic
+modes, and exact mode stays bit-identical to the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from wind_amd import api
+from wind_amd import networks as W
+
+pytestmark = pytest.mark.gpu
+
+
+def run(net, y0, c, nsys, tend, tol, mode, hint=None, device=False):
+    steps = np.zeros(nsys, np.int32)
+    status = np.zeros(nsys, np.int32)
+    if device:
+        import torch
+
+        dev = torch.device("cuda:0")
+        y = torch.from_numpy(y0).to(dev)
+        cd = torch.from_numpy(c).to(dev)
+        sd = torch.zeros(nsys, dtype=torch.int32, device=dev)
+        std = torch.zeros(nsys, dtype=torch.int32, device=dev)
+        hd = None if hint is None else torch.from_numpy(hint).to(dev)
+        total, _ = api.integrate(net, 0.0, tend, 1, cd, y, nsys, tol, tol, method=api.SIE, mode=mode,
+                                 kernel=api.KERNEL_LANE, device_pointers=True, steps_out=sd, status_out=std,
+                                 controller=api.CTRL_ADAPTIVE, cost_hint=hd)
+        torch.cuda.synchronize()
+        return y.cpu().numpy(), sd.cpu().numpy(), std.cpu().numpy(), total
+    y = y0.copy()
+    total, _ = api.integrate(net, 0.0, tend, 1, c, y, nsys, tol, tol, method=api.SIE, mode=mode,
+                             kernel=api.KERNEL_LANE, steps_out=steps, status_out=status,
+                             controller=api.CTRL_ADAPTIVE, cost_hint=hint)
+    return y, steps, status, total
+
+
+def same(a, b, label):
+    assert np.array_equal(a[2], b[2]), label
+    assert np.array_equal(a[1], b[1]), label
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)), label
+    assert a[3] == b[3], label
+
+
+@pytest.mark.parametrize("device", [False, True])
+def test_hhe6_fast_hinted_order_is_bit_identical(device):
+    """2^18 hhe6 systems, fast mode, adaptive controller at tol 1e-5: the
+    hint is the unhinted run's steps_out, and a random hint."""
+    n = 1 << 18
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(n, seed=7)
+    net = api.Network.builtin(api.NET_HHE6, p)
+    base = run(net, y0, nH, n, W.HHE6_TEND_YR, 1e-5, api.MODE_FAST, device=device)
+    assert (base[2] == 0).all()
+    same(run(net, y0, nH, n, W.HHE6_TEND_YR, 1e-5, api.MODE_FAST, hint=base[1].copy(), device=device), base,
+         "steps_out hint")
+    rnd = np.random.default_rng(8).integers(-5, 1 << 30, n, dtype=np.int64).astype(np.int32)
+    same(run(net, y0, nH, n, W.HHE6_TEND_YR, 1e-5, api.MODE_FAST, hint=rnd, device=device), base, "random hint")
+
+
+def test_exact_mode_hinted_matches_oracle():
+    """Exact arithmetic with a hint still reproduces the oracle bit for bit."""
+    n = 256
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(n, seed=9)
+    net = api.Network.builtin(api.NET_HHE6, p)
+    hint = np.arange(n, dtype=np.int32)[::-1].copy()
+    g = run(net, y0, nH, n, 10.0, 1e-3, api.MODE_EXACT, hint=hint)
+    yr = y0.copy()
+    total, steps, status = O.gold_integrate(api.SIE, O.hhe6(p), 0.0, 10.0, 1, nH, yr, n, 1e-3, 1e-3,
+                                            const_stride=1, controller=api.CTRL_ADAPTIVE)
+    assert np.array_equal(g[2], status)
+    assert np.array_equal(g[1], steps)
+    assert np.array_equal(g[0].view(np.uint32), yr.view(np.uint32))
+    assert g[3] == total

@@ -40,7 +40,7 @@ class WindOptions(C.Structure):
         ("device_pointers", C.c_int), ("max_attempts", C.c_int64),
         ("steps_out", C.c_void_p), ("status_out", C.c_void_p), ("total_steps", C.c_void_p),
         ("stream", C.c_void_p), ("kernel_ms", C.c_void_p), ("controller", C.c_int),
-        ("solver", C.c_int), ("ir_sweeps", C.c_int), ("flags", C.c_int),
+        ("solver", C.c_int), ("ir_sweeps", C.c_int), ("flags", C.c_int), ("cost_hint", C.c_void_p),
     ]
 
 
@@ -190,8 +190,10 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
               absolute, relative, method=SIE, mode=MODE_EXACT, kernel=KERNEL_AUTO,
               const_stride=0, device_pointers=False, max_attempts=0, steps_out=None,
               status_out=None, stream=None, want_kernel_ms=False, controller=CTRL_REFERENCE,
-              solver=SOLVER_AUTO, ir_sweeps=0):
-    """Integrates `nsys` systems in place.  Returns (total_substeps, kernel_ms)."""
+              solver=SOLVER_AUTO, ir_sweeps=0, cost_hint=None):
+    """Integrates `nsys` systems in place.  Returns (total_substeps, kernel_ms).
+    `cost_hint`: optional int32 per-system cost estimates (e.g. a previous
+    call's steps_out); lane kernels start systems in decreasing order of it."""
     lib = load()
     o = WindOptions()
     lib.wind_options_init(C.byref(o))
@@ -208,6 +210,7 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
         o.kernel_ms = C.cast(C.byref(kms), C.c_void_p)
     o.steps_out = _ptr(steps_out)
     o.status_out = _ptr(status_out)
+    o.cost_hint = _ptr(cost_hint)
     o.stream = stream
     if not device_pointers:
         for a, dt in ((constants, np.float32), (equations, np.float32)):

@@ -71,6 +71,7 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Pool constants, equations, steps, status, counters;
+    Pool hint, order;  // cost hints copied from the host; start order and class counts
     // batch execution
     std::vector<hipStream_t> bucket_streams;
     std::vector<hipEvent_t> bucket_events;
@@ -82,6 +83,66 @@ struct Context {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
 };
+
+// Longest-first start order from per-system cost hints: a counting sort
+// into classes of four per octave of the hint, larger classes first (a
+// non-positive hint is class 0, last).  Order within a class is arbitrary;
+// the lane kernels take systems in the resulting order.
+constexpr int kHintClasses = 128;
+constexpr int kHintBlocks = 1024;
+
+__device__ inline int hint_class(int32_t h) {
+    if (h <= 0) return 0;
+    const unsigned u = (unsigned)h;
+    const int lg = 31 - __clz(u);
+    const int sub = lg >= 2 ? (int)((u >> (lg - 2)) & 3u) : (int)((u << (2 - lg)) & 3u);
+    return 1 + 4 * lg + sub;
+}
+
+// Each block counts the classes of its contiguous chunk of systems.
+__global__ void hint_count(const int32_t *hint, int n, unsigned *count) {
+    __shared__ unsigned h[kHintClasses];
+    for (int i = threadIdx.x; i < kHintClasses; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[hint_class(hint[i])], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHintClasses; i += blockDim.x)
+        if (h[i]) atomicAdd(&count[i], h[i]);
+}
+
+// Class counts to start positions, the largest class first (one thread).
+__global__ void hint_starts(unsigned *count) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned run = 0;
+    for (int c = kHintClasses - 1; c >= 0; c--) {
+        const unsigned k = count[c];
+        count[c] = run;
+        run += k;
+    }
+}
+
+// Each block reserves its chunk's range in every class, then places its
+// systems there.
+__global__ void hint_place(const int32_t *hint, int n, unsigned *start, int32_t *index) {
+    __shared__ unsigned h[kHintClasses], base[kHintClasses];
+    for (int i = threadIdx.x; i < kHintClasses; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[hint_class(hint[i])], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHintClasses; i += blockDim.x) {
+        base[i] = h[i] ? atomicAdd(&start[i], h[i]) : 0u;
+        h[i] = 0;
+    }
+    __syncthreads();
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const int c = hint_class(hint[i]);
+        index[base[c] + atomicAdd(&h[c], 1u)] = i;
+    }
+}
 
 __global__ void zero_u64(unsigned long long *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
@@ -499,6 +560,34 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     if (rc) return rc;
     WIND_HIP(hipMemsetAsync(c.counters.ptr, 0, 4 * sizeof(unsigned long long), st));
 
+    // Start order from cost hints, for the kernels that take systems from a
+    // queue (the lane kernels of compiled-in and generated networks).
+    const int32_t *d_index = nullptr;
+    const bool lane_queue = (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
+                             net->builtin == WIND_NET_HHE6 || net->builtin == WIND_NET_PLUGIN) &&
+                            opts.kernel != WIND_KERNEL_BLOCK;
+    if (opts.cost_hint && lane_queue) {
+        const int32_t *d_hint = opts.cost_hint;
+        if (!opts.device_pointers) {
+            rc = reserve(c.hint, (size_t)Nsystems * 4);
+            if (rc) return rc;
+            WIND_HIP(hipMemcpyAsync(c.hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
+            d_hint = (const int32_t *)c.hint.ptr;
+        }
+        const size_t index_bytes = ((size_t)Nsystems * 4 + 255) & ~(size_t)255;
+        rc = reserve(c.order, index_bytes + kHintClasses * sizeof(unsigned));
+        if (rc) return rc;
+        int32_t *index = (int32_t *)c.order.ptr;
+        unsigned *count = (unsigned *)((char *)c.order.ptr + index_bytes);
+        const int blocks = std::min(kHintBlocks, (Nsystems + 255) / 256);
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)count, kHintClasses / 2);
+        hipLaunchKernelGGL(hint_count, dim3(blocks), dim3(256), 0, st, d_hint, Nsystems, count);
+        hipLaunchKernelGGL(hint_starts, dim3(1), dim3(64), 0, st, count);
+        hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, d_hint, Nsystems, count, index);
+        WIND_HIP(hipGetLastError());
+        d_index = index;
+    }
+
     IntegrateArgs a;
     a.tnow = tnow;
     a.tend = tend;
@@ -513,7 +602,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     a.max_attempts = opts.max_attempts > 0 ? opts.max_attempts : kDefaultMaxAttempts;
     a.steps_out = d_steps;
     a.status_out = d_status;
-    a.index = nullptr;
+    a.index = d_index;
     a.counters = (unsigned long long *)c.counters.ptr;
 
     WIND_HIP(hipEventRecord(c.ev0, st));
