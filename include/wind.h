@@ -162,8 +162,10 @@ typedef struct wind_options {
     int solver;            /* wind_solver (fast mode, mass-action networks) */
     int ir_sweeps;         /* iterative-refinement sweeps after the solve (0-4) */
     int flags;             /* WIND_FLAG_* */
-    const int32_t *cost_hint; /* wind_integrate, lane kernels (compiled-in and
-                              generated networks): optional [Nsystems]
+    const int32_t *cost_hint; /* wind_integrate, kernels with a work queue (the
+                              lane kernels of compiled-in and generated
+                              networks, the dense solver of mass-action
+                              networks in fast mode): optional [Nsystems]
                               per-system cost estimates, device or host as
                               the other arrays (e.g. the previous call's
                               steps_out for the same cells).  Systems start in

@@ -79,3 +79,22 @@ def test_exact_mode_hinted_matches_oracle():
     assert np.array_equal(g[1], steps)
     assert np.array_equal(g[0].view(np.uint32), yr.view(np.uint32))
     assert g[3] == total
+
+
+def test_mass_action_dense_hinted_order_is_bit_identical():
+    """The dense solver (mass action, N = 64, fast mode) takes systems from
+    its queue in the hinted order; results are bit-identical."""
+    b = [b for b in W.mixed_workload(1 << 14, seed=4) if b["N"] == 64][0]
+    n = min(1024, len(b["scale"]))
+    net = api.Network.mass_action(64, b["rx"], b["k"])
+    out = []
+    for hint in (None, "steps"):
+        y = b["y0"].reshape(-1, 64)[:n].reshape(-1).copy()
+        steps = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        h = None if hint is None else out[0][1][::-1].copy()
+        total, _ = api.integrate(net, 0.0, 1e9, 1, b["scale"][:n].copy(), y, n, 1e-10, 8e-5, mode=api.MODE_FAST,
+                                 controller=api.CTRL_ADAPTIVE, steps_out=steps, status_out=status,
+                                 max_attempts=100000, cost_hint=h)
+        out.append((y, steps, status, total))
+    same(out[1], out[0], "dense N=64")

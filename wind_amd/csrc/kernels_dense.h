@@ -525,7 +525,10 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
     unsigned long long steps = 0;
     unsigned int *head = reinterpret_cast<unsigned int *>(a.counters);
     for (;;) {
-        if (t == 0) s_sid = (int)atomicAdd(head, 1u);
+        if (t == 0) {
+            const unsigned q = atomicAdd(head, 1u);
+            s_sid = q < (unsigned)a.nsys ? (a.index ? a.index[q] : (int)q) : a.nsys;
+        }
         __syncthreads();
         const int sid = s_sid;
         if (sid >= a.nsys) break;

@@ -561,12 +561,14 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     WIND_HIP(hipMemsetAsync(c.counters.ptr, 0, 4 * sizeof(unsigned long long), st));
 
     // Start order from cost hints, for the kernels that take systems from a
-    // queue (the lane kernels of compiled-in and generated networks).
+    // queue: the lane kernels of compiled-in and generated networks and the
+    // dense solver of mass-action networks (fast mode).
     const int32_t *d_index = nullptr;
-    const bool lane_queue = (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
-                             net->builtin == WIND_NET_HHE6 || net->builtin == WIND_NET_PLUGIN) &&
-                            opts.kernel != WIND_KERNEL_BLOCK;
-    if (opts.cost_hint && lane_queue) {
+    const bool queue_kernel = opts.kernel != WIND_KERNEL_BLOCK &&
+                              (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
+                               net->builtin == WIND_NET_HHE6 || net->builtin == WIND_NET_PLUGIN ||
+                               (net->builtin == WIND_NET_MASS_ACTION && opts.mode == WIND_MODE_FAST));
+    if (opts.cost_hint && queue_kernel) {
         const int32_t *d_hint = opts.cost_hint;
         if (!opts.device_pointers) {
             rc = reserve(c.hint, (size_t)Nsystems * 4);
