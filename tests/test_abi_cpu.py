@@ -94,3 +94,25 @@ def test_mass_action_table_validation():
         api.Network.mass_action(4, np.array([0, 1, 2, 9], np.int32), np.ones(1, np.float32))
     with pytest.raises(api.WindError, match="N out of range"):
         api.Network.mass_action(0, np.zeros(4, np.int32), np.ones(1, np.float32))
+
+
+def test_options_struct_matches_header():
+    """api.WindOptions (ctypes) has the members of wind_options in
+    include/wind.h, in order and with matching kinds (pointer, int, int64),
+    so wind_options_init and wind_integrate read what Python writes."""
+    text = open(os.path.join(ROOT, "include", "wind.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct wind_options\s*\{(.*?)\}\s*wind_options;", text, flags=re.S).group(1)
+    members = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        ctype, name = decl.rsplit(" ", 1)
+        ptr = "*" in ctype or name.startswith("*")
+        name = name.lstrip("*")
+        kind = "ptr" if ptr else ("int64" if "int64_t" in ctype else "int")
+        members.append((name, kind))
+    kinds = {C.c_void_p: "ptr", C.c_int64: "int64", C.c_int: "int"}
+    fields = [(n, kinds[t]) for n, t in api.WindOptions._fields_]
+    assert fields == members
