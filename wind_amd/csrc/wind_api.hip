@@ -72,6 +72,7 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Pool constants, equations, steps, status, counters;
     Pool hint, order;  // cost hints copied from the host; start order and class counts
+    Pool out_steps;    // output-step driver: per-system sub-steps of the last interval
     // batch execution
     std::vector<hipStream_t> bucket_streams;
     std::vector<hipEvent_t> bucket_events;
@@ -175,6 +176,33 @@ int reserve(Pool &p, size_t bytes) {
     WIND_HIP(hipMalloc(&p.ptr, want));
     p.bytes = want;
     return WIND_OK;
+}
+
+// Writes the longest-first start order of `n` systems from device hints
+// into pool `order` (the permutation, then the class counts) on stream st.
+int order_by_hint(Pool &order, const int32_t *d_hint, int n, hipStream_t st, const int32_t **index_out) {
+    const size_t index_bytes = ((size_t)n * 4 + 255) & ~(size_t)255;
+    const int rc = reserve(order, index_bytes + kHintClasses * sizeof(unsigned));
+    if (rc) return rc;
+    int32_t *index = (int32_t *)order.ptr;
+    unsigned *count = (unsigned *)((char *)order.ptr + index_bytes);
+    const int blocks = std::min(kHintBlocks, (n + 255) / 256);
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)count, kHintClasses / 2);
+    hipLaunchKernelGGL(hint_count, dim3(blocks), dim3(256), 0, st, d_hint, n, count);
+    hipLaunchKernelGGL(hint_starts, dim3(1), dim3(64), 0, st, count);
+    hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, d_hint, n, count, index);
+    WIND_HIP(hipGetLastError());
+    *index_out = index;
+    return WIND_OK;
+}
+
+// Whether the kernels that wind_integrate would launch take systems from a
+// work queue (and so follow a start order): the lane kernels of compiled-in
+// and generated networks and the dense solver of mass-action networks.
+bool queue_kernel(const wind_network *net, const wind_options &o) {
+    return o.kernel != WIND_KERNEL_BLOCK &&
+           (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 || net->builtin == WIND_NET_HHE6 ||
+            net->builtin == WIND_NET_PLUGIN || (net->builtin == WIND_NET_MASS_ACTION && o.mode == WIND_MODE_FAST));
 }
 
 template <class K>
@@ -430,6 +458,30 @@ int wind_integrate_outputs(const wind_network *net, double tnow, double tend, in
     }
     unsigned long long *cnt = (unsigned long long *)c.out_counters.ptr;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, cnt, n_output_steps * 4);
+    // Kernels with a work queue start each interval's systems longest first,
+    // by their sub-step counts in the interval before (the first interval by
+    // the caller's cost hints, if given); results do not depend on the order.
+    const bool ordered = queue_kernel(net, opts);
+    int32_t *d_steps = nullptr;
+    const int32_t *d_index = nullptr;
+    bool have_steps = false;
+    if (ordered) {
+        rc = reserve(c.out_steps, (size_t)Nsystems * 4);
+        if (rc) return rc;
+        d_steps = (int32_t *)c.out_steps.ptr;
+        if (opts.cost_hint) {
+            const int32_t *d_hint = opts.cost_hint;
+            if (!opts.device_pointers) {
+                rc = reserve(c.hint, (size_t)Nsystems * 4);
+                if (rc) return rc;
+                WIND_HIP(hipMemcpyAsync(c.hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice,
+                                        st));
+                d_hint = (const int32_t *)c.hint.ptr;
+            }
+            rc = order_by_hint(c.order, d_hint, Nsystems, st, &d_index);
+            if (rc) return rc;
+        }
+    }
     SolverChoice sc;
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
@@ -453,13 +505,18 @@ int wind_integrate_outputs(const wind_network *net, double tnow, double tend, in
             a.nsys = Nsystems;
             a.const_stride = stride;
             a.max_attempts = opts.max_attempts > 0 ? opts.max_attempts : kDefaultMaxAttempts;
-            a.steps_out = nullptr;
+            if (have_steps) {
+                rc = order_by_hint(c.order, d_steps, Nsystems, st, &d_index);
+                if (rc) return rc;
+            }
+            a.steps_out = d_steps;
             a.status_out = nullptr;
-            a.index = nullptr;
+            a.index = d_index;
             a.counters = cnt + 4 * (k - 1);
             rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, opts.mode, opts.kernel, sc, st)
                                          : dispatch<WIND_RK2>(c, net, a, opts.mode, opts.kernel, sc, st);
             if (rc) return rc;
+            have_steps = ordered;
         }
         WIND_HIP(hipEventRecord(c.out_events[k], st));
     }
@@ -560,15 +617,9 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     if (rc) return rc;
     WIND_HIP(hipMemsetAsync(c.counters.ptr, 0, 4 * sizeof(unsigned long long), st));
 
-    // Start order from cost hints, for the kernels that take systems from a
-    // queue: the lane kernels of compiled-in and generated networks and the
-    // dense solver of mass-action networks (fast mode).
+    // Start order from cost hints, for the kernels that take systems from a queue.
     const int32_t *d_index = nullptr;
-    const bool queue_kernel = opts.kernel != WIND_KERNEL_BLOCK &&
-                              (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
-                               net->builtin == WIND_NET_HHE6 || net->builtin == WIND_NET_PLUGIN ||
-                               (net->builtin == WIND_NET_MASS_ACTION && opts.mode == WIND_MODE_FAST));
-    if (opts.cost_hint && queue_kernel) {
+    if (opts.cost_hint && queue_kernel(net, opts)) {
         const int32_t *d_hint = opts.cost_hint;
         if (!opts.device_pointers) {
             rc = reserve(c.hint, (size_t)Nsystems * 4);
@@ -576,18 +627,8 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
             WIND_HIP(hipMemcpyAsync(c.hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
             d_hint = (const int32_t *)c.hint.ptr;
         }
-        const size_t index_bytes = ((size_t)Nsystems * 4 + 255) & ~(size_t)255;
-        rc = reserve(c.order, index_bytes + kHintClasses * sizeof(unsigned));
+        rc = order_by_hint(c.order, d_hint, Nsystems, st, &d_index);
         if (rc) return rc;
-        int32_t *index = (int32_t *)c.order.ptr;
-        unsigned *count = (unsigned *)((char *)c.order.ptr + index_bytes);
-        const int blocks = std::min(kHintBlocks, (Nsystems + 255) / 256);
-        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)count, kHintClasses / 2);
-        hipLaunchKernelGGL(hint_count, dim3(blocks), dim3(256), 0, st, d_hint, Nsystems, count);
-        hipLaunchKernelGGL(hint_starts, dim3(1), dim3(64), 0, st, count);
-        hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, d_hint, Nsystems, count, index);
-        WIND_HIP(hipGetLastError());
-        d_index = index;
     }
 
     IntegrateArgs a;
