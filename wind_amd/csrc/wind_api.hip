@@ -88,7 +88,7 @@ struct Context {
 // Longest-first start order from per-system cost hints: a counting sort
 // into classes of four per octave of the hint, larger classes first (a
 // non-positive hint is class 0, last).  Order within a class is arbitrary;
-// the lane kernels take systems in the resulting order.
+// the kernels with a work queue take systems in the resulting order.
 constexpr int kHintClasses = 128;
 constexpr int kHintBlocks = 1024;
 
