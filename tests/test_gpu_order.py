@@ -98,3 +98,28 @@ def test_mass_action_dense_hinted_order_is_bit_identical():
                                  max_attempts=100000, cost_hint=h)
         out.append((y, steps, status, total))
     same(out[1], out[0], "dense N=64")
+
+
+def test_legacy_symbol_repeated_calls_are_bit_identical():
+    """WIND's output loop calls the drop-in symbol on one state array per
+    output step; from the second call on, the shim starts systems longest
+    first by the call before's sub-step counts (2^18 systems, above its
+    threshold).  States and returned sub-step totals equal those of
+    unhinted calls over the same intervals."""
+    import ctypes as C
+    import os
+
+    n = 1 << 18
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    so = C.CDLL(os.path.join(root, "wind_amd", "lib", "hhe6", "sie.so"))
+    fn = getattr(so, "_Z19cudaIntegrateSystemffiPfS_iiff")
+    fn.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float]
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(n, seed=10)
+    net = api.Network.builtin(api.NET_HHE6, p)
+    ys, yr = y0.copy(), y0.copy()
+    for t0, t1 in ((0.0, 1e3), (1e3, 2e3), (2e3, 3e3)):
+        got = fn(t0, t1, 1, nH.ctypes.data, ys.ctypes.data, n, 6, 1e-3, 1e-3)
+        total, _ = api.integrate(net, t0, t1, 1, nH, yr, n, 1e-3, 1e-3, method=api.SIE, mode=api.MODE_EXACT)
+        assert got == (total & 0xFFFFFFFF) - (1 << 32 if total & 0x80000000 else 0)
+        assert np.array_equal(ys.view(np.uint32), yr.view(np.uint32)), (t0, t1)
