@@ -123,3 +123,26 @@ def test_legacy_symbol_repeated_calls_are_bit_identical():
         total, _ = api.integrate(net, t0, t1, 1, nH, yr, n, 1e-3, 1e-3, method=api.SIE, mode=api.MODE_EXACT)
         assert got == (total & 0xFFFFFFFF) - (1 << 32 if total & 0x80000000 else 0)
         assert np.array_equal(ys.view(np.uint32), yr.view(np.uint32)), (t0, t1)
+
+
+def test_generated_network_hinted_order_is_bit_identical():
+    """A generated network (plugin lane kernel, N = 5, fast mode) follows the
+    hinted order with bit-identical results."""
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    plugin = os.path.join(root, "wind_amd", "lib", "gen", "mixed5", "libwind_net.so")
+    b = [b for b in W.mixed_workload(1 << 14, seed=4) if b["N"] == 5][0]
+    n = min(4096, len(b["scale"]))
+    net = api.Network.plugin(plugin)
+    out = []
+    for k in range(2):
+        y = b["y0"].reshape(-1, 5)[:n].reshape(-1).copy()
+        steps = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        h = None if k == 0 else out[0][1].copy()
+        total, _ = api.integrate(net, 0.0, 1e9, 1, b["scale"][:n].copy(), y, n, 1e-10, 8e-5, mode=api.MODE_FAST,
+                                 controller=api.CTRL_ADAPTIVE, steps_out=steps, status_out=status,
+                                 max_attempts=100000, cost_hint=h)
+        out.append((y, steps, status, total))
+    same(out[1], out[0], "generated N=5")
