@@ -292,3 +292,27 @@ def test_server_resident_block_advances_in_order():
     srv.read_block(blk, out)
     srv.close()
     assert np.array_equal(out, y)
+
+
+def test_server_large_block_ordered_requests_match_calls():
+    """A resident block of 2^18 systems keeps its systems' sub-step counts,
+    and each later request on it starts them longest first; the states after
+    three requests equal those of three plain calls (the order changes no
+    result)."""
+    hp = W.hhe6_params()
+    net = api.Network.builtin(api.NET_HHE6, hp)
+    n = 1 << 18
+    y0, c = W.hhe6_systems(n, seed=12)
+    spans = ((0.0, 1e3), (1e3, 1e4), (1e4, 1e5))
+    y = y0.copy()
+    for (t0, t1) in spans:
+        api.integrate(net, t0, t1, 1, c, y, n, 1e-5, 1e-5, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE)
+    srv = api.Server(256 << 20, nslots=2)
+    blk = srv.block(net, c, y0, n)
+    ids = [srv.submit(t0, t1, 1e-5, 1e-5, block=blk) for (t0, t1) in spans]
+    for i in ids:
+        srv.wait(i)
+    out = np.empty_like(y0)
+    srv.read_block(blk, out)
+    srv.close()
+    assert np.array_equal(out.view(np.uint32), y.view(np.uint32))

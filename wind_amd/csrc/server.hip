@@ -96,11 +96,20 @@ struct Slot {
 
 }  // namespace
 
+// A block of at least kBlockOrderMin systems also keeps its systems'
+// sub-step counts from the last request and a start order: the next
+// request on it starts them longest first (results do not depend on the
+// order; smaller blocks start every system at once anyway).
+constexpr int kBlockOrderMin = 1 << 17;
+
 struct wind_block {
     const wind_network *net = nullptr;
     int nsys = 0, const_stride = 0;
     size_t off = 0, bytes = 0;
     float *constants = nullptr, *equations = nullptr;  // device (in the arena)
+    int32_t *steps = nullptr, *order = nullptr;        // device, [nsys] each, or null
+    unsigned *count = nullptr;                         // device, kHintCountWords
+    bool have_steps = false;
     bool busy = false;
 };
 
@@ -212,6 +221,14 @@ bool start(wind_server *srv, Request &r, int si) {
     a.status_out = nullptr;
     a.index = nullptr;
     a.counters = s.counters;
+    const bool ordered = r.d.block && r.d.block->steps && wind_queue_kernel(net, r.d.mode, WIND_KERNEL_AUTO);
+    if (ordered) {
+        if (r.d.block->have_steps) {
+            wind_order_by_hint(r.d.block->steps, nsys, r.d.block->order, r.d.block->count, s.stream);
+            a.index = r.d.block->order;
+        }
+        a.steps_out = r.d.block->steps;
+    }
     // Full-device grid (measured against a fixed 1/nslots share of the
     // device: more requests per second and lower p50 and p99 latency).
     SolverChoice sc;
@@ -222,6 +239,7 @@ bool start(wind_server *srv, Request &r, int si) {
             if (r.d.block) r.d.block->busy = false;
             return false;
         }
+        if (ordered) r.d.block->have_steps = true;
     }
     s.out_off = cba;
     s.out_bytes = r.d.equations ? eb : 0;
@@ -360,7 +378,9 @@ int wind_server_block_create(wind_server *srv, const wind_network *net, const fl
     b->const_stride = const_stride > 0 ? const_stride : net->nconst;
     const size_t cb = ((size_t)(nsys - 1) * b->const_stride + net->nconst) * sizeof(float);
     const size_t eb = (size_t)nsys * net->N * sizeof(float);
-    b->bytes = ((cb + 255) & ~(size_t)255) + eb;
+    const size_t ebp = (eb + 255) & ~(size_t)255;
+    const size_t ib = nsys >= kBlockOrderMin ? (((size_t)nsys * 4 + 255) & ~(size_t)255) : 0;
+    b->bytes = ((cb + 255) & ~(size_t)255) + ebp + 2 * ib + (ib ? kHintCountWords * sizeof(unsigned) : 0);
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         if (!srv->arena.alloc(b->bytes, &b->off)) {
@@ -370,6 +390,12 @@ int wind_server_block_create(wind_server *srv, const wind_network *net, const fl
     }
     b->constants = (float *)(srv->arena_base + b->off);
     b->equations = (float *)(srv->arena_base + b->off + ((cb + 255) & ~(size_t)255));
+    if (ib) {
+        char *extra = (char *)b->equations + ebp;
+        b->steps = (int32_t *)extra;
+        b->order = (int32_t *)(extra + ib);
+        b->count = (unsigned *)(extra + 2 * ib);
+    }
     if (hipMemcpy(b->constants, constants, cb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->equations, equations, eb, hipMemcpyHostToDevice) != hipSuccess) {
         std::lock_guard<std::mutex> lk(srv->mu);

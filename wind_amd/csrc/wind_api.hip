@@ -185,25 +185,13 @@ int order_by_hint(Pool &order, const int32_t *d_hint, int n, hipStream_t st, con
     const int rc = reserve(order, index_bytes + kHintClasses * sizeof(unsigned));
     if (rc) return rc;
     int32_t *index = (int32_t *)order.ptr;
-    unsigned *count = (unsigned *)((char *)order.ptr + index_bytes);
-    const int blocks = std::min(kHintBlocks, (n + 255) / 256);
-    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)count, kHintClasses / 2);
-    hipLaunchKernelGGL(hint_count, dim3(blocks), dim3(256), 0, st, d_hint, n, count);
-    hipLaunchKernelGGL(hint_starts, dim3(1), dim3(64), 0, st, count);
-    hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, d_hint, n, count, index);
+    wind_order_by_hint(d_hint, n, index, (unsigned *)((char *)order.ptr + index_bytes), st);
     WIND_HIP(hipGetLastError());
     *index_out = index;
     return WIND_OK;
 }
 
-// Whether the kernels that wind_integrate would launch take systems from a
-// work queue (and so follow a start order): the lane kernels of compiled-in
-// and generated networks and the dense solver of mass-action networks.
-bool queue_kernel(const wind_network *net, const wind_options &o) {
-    return o.kernel != WIND_KERNEL_BLOCK &&
-           (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 || net->builtin == WIND_NET_HHE6 ||
-            net->builtin == WIND_NET_PLUGIN || (net->builtin == WIND_NET_MASS_ACTION && o.mode == WIND_MODE_FAST));
-}
+bool queue_kernel(const wind_network *net, const wind_options &o) { return wind_queue_kernel(net, o.mode, o.kernel); }
 
 template <class K>
 int persistent_grid(const Context &c, K kernel, int block, int nsys, int *grid, const SolverChoice &sc) {
@@ -942,6 +930,23 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
     for (int i = 0; i < nb; i++) total += (long long)host[4 * i + 1];
     if (o.total_steps) *o.total_steps = total;
     return WIND_OK;
+}
+
+void wind_order_by_hint(const int32_t *hint, int n, int32_t *index, unsigned *count, hipStream_t st) {
+    static_assert(kHintCountWords == kHintClasses, "one count per class");
+    const int blocks = std::min(kHintBlocks, (n + 255) / 256);
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)count, kHintClasses / 2);
+    hipLaunchKernelGGL(hint_count, dim3(blocks), dim3(256), 0, st, hint, n, count);
+    hipLaunchKernelGGL(hint_starts, dim3(1), dim3(64), 0, st, count);
+    hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, hint, n, count, index);
+}
+
+// The lane kernels of compiled-in and generated networks and the dense
+// solver of mass-action networks (fast mode) take systems from a queue.
+bool wind_queue_kernel(const wind_network *net, int mode, int kernel) {
+    return kernel != WIND_KERNEL_BLOCK &&
+           (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 || net->builtin == WIND_NET_HHE6 ||
+            net->builtin == WIND_NET_PLUGIN || (net->builtin == WIND_NET_MASS_ACTION && mode == WIND_MODE_FAST));
 }
 
 int wind_launch_async(const wind_network *net, const IntegrateArgs &a, int method, int mode, int kernel,

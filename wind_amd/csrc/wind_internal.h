@@ -101,3 +101,10 @@ int wind_launch_async(const wind_network *net, const wind::IntegrateArgs &a, int
 int wind_device_cus(int *num_cu);
 // Fills n 64-bit words with zero on `st` (graph- and capture-safe).
 void wind_zero_counters(unsigned long long *p, int n, hipStream_t st);
+// Longest-first start order of n systems from device cost hints, written to
+// index[n] on `st`; count is device scratch of kHintCountWords 32-bit words.
+constexpr int kHintCountWords = 128;
+void wind_order_by_hint(const int32_t *hint, int n, int32_t *index, unsigned *count, hipStream_t st);
+// Whether the kernels launched for (net, mode, kernel) take systems from a
+// work queue, so that a start order applies.
+bool wind_queue_kernel(const wind_network *net, int mode, int kernel);
