@@ -10,7 +10,9 @@
 // workgroups of requests launched after it.  Completion is
 // observed with an event; latency runs from submission to results on the
 // host.  Resident blocks keep a cell block's state in the arena across
-// requests (only results travel); requests on one block run in order.
+// requests (only results travel); requests on one block run in order, and
+// a large block starts each request's systems longest first by their
+// sub-step counts in the request before.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
