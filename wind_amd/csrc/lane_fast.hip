@@ -4,6 +4,8 @@
 // exact mode (wind_api.hip, contraction off) keeps the reference's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels_small.h"
 #include "wind_internal.h"
 
@@ -21,7 +23,73 @@ __global__ void __launch_bounds__(256, WIND_LANE_FAST_MIN_BLOCKS)
     lane_integrate_body<Net, METHOD, MathFast>(a, p);
 }
 
+// Start-order estimate from the initial state alone: x = (tend - tnow) *
+// max_i |f_i(y0)| / (|y0_i| + ABS), the fastest relative rate of change
+// over the interval, one right-hand side per system.  Its float bits go to
+// xbits[s] (non-negative floats order as their bit patterns) and the
+// largest to *xmax.
+template <class Net>
+__global__ void __launch_bounds__(256) lane_activity(IntegrateArgs a, typename Net::Params p, unsigned *xbits,
+                                                      unsigned *xmax) {
+    constexpr int N = Net::N;
+    constexpr int NC = Net::NCONST;
+    const float span = a.tend - a.tnow;
+    const float floor_abs = a.ABS > 0.0f ? a.ABS : 1.17549435e-38f;
+    unsigned best = 0u;
+    for (int sid = blockIdx.x * blockDim.x + threadIdx.x; sid < a.nsys; sid += gridDim.x * blockDim.x) {
+        float y[N], c[NC], f[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) y[i] = a.equations[(size_t)sid * N + i];
+#pragma unroll
+        for (int k = 0; k < NC; k++) c[k] = a.constants[(size_t)sid * a.const_stride + k];
+        Net::template rhs<MathFast>(p, c, y, f);
+        float r = 0.0f;
+#pragma unroll
+        for (int i = 0; i < N; i++) r = fmaxf(r, fabsf(f[i]) / (fabsf(y[i]) + floor_abs));
+        float x = r * span;
+        if (!(x >= 0.0f)) x = 0.0f;                   // NaN
+        if (x > 3.0e38f) x = 3.0e38f;                 // inf
+        const unsigned b = __float_as_uint(x);
+        xbits[sid] = b;
+        best = b > best ? b : best;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(best, off);
+        best = o > best ? o : best;
+    }
+    if (__lane_id() == 0 && best) atomicMax(xmax, best);
+}
+
+// Turns the estimates into cost hints whose classes (four per octave, as
+// the start-order sort forms them) keep 29 octaves below the largest: an
+// estimate e octaves below it gets class 124 - floor(4 e) (at least 9), and
+// the hint is the smallest integer of that class.
+__global__ void __launch_bounds__(256) lane_activity_hints(const unsigned *xbits, int n, const unsigned *xmax,
+                                                            int32_t *hint) {
+    const float lmax = __log2f(fmaxf(__uint_as_float(*xmax), 1.0e-30f));
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const float l = __log2f(fmaxf(__uint_as_float(xbits[s]), 1.0e-30f));
+        const float below = fminf(fmaxf(4.0f * (lmax - l), 0.0f), 115.0f);
+        const int cls = 124 - (int)below;             // 9 .. 124
+        const int lg = (cls - 1) >> 2, sub = (cls - 1) & 3;
+        hint[s] = (4 + sub) << (lg - 2);
+    }
+}
+
 }  // namespace
+
+template <class Net>
+int launch_lane_activity(int num_cu, const typename Net::Params &p, const IntegrateArgs &a, int32_t *hint,
+                         unsigned *scratch, hipStream_t st) {
+    // scratch: [0] the largest estimate, then one word per system
+    unsigned *xmax = scratch, *xbits = scratch + 64;
+    const int grid = std::min(num_cu * 8, (a.nsys + 255) / 256);
+    hipLaunchKernelGGL(lane_activity<Net>, dim3(grid), dim3(256), 0, st, a, p, xbits, xmax);
+    hipLaunchKernelGGL(lane_activity_hints, dim3(grid), dim3(256), 0, st, xbits, a.nsys, xmax, hint);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WIND_OK : wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+}
 
 // Persistent blocks per CU.  Fewer lanes than the registers allow run
 // faster: on the 2^20-system hhe6 workload (SIE, tol 2e-6) a launch took
@@ -44,6 +112,8 @@ int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateA
 }
 
 #define WIND_LANE_FAST(NET)                                                                                   \
+    template int launch_lane_activity<NET>(int, const NET::Params &, const IntegrateArgs &, int32_t *,         \
+                                           unsigned *, hipStream_t);                                          \
     template int launch_lane_fast<NET, WIND_SIE>(int, const NET::Params &, const IntegrateArgs &,             \
                                                  const SolverChoice &, hipStream_t);                          \
     template int launch_lane_fast<NET, WIND_RK2>(int, const NET::Params &, const IntegrateArgs &,             \

@@ -72,6 +72,7 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Pool constants, equations, steps, status, counters;
     Pool hint, order;  // cost hints copied from the host; start order and class counts
+    Pool activity;     // initial-state work estimates (start order without caller hints)
     Pool out_steps;    // output-step driver: per-system sub-steps of the last interval
     // batch execution
     std::vector<hipStream_t> bucket_streams;
@@ -192,6 +193,30 @@ int order_by_hint(Pool &order, const int32_t *d_hint, int n, hipStream_t st, con
 }
 
 bool queue_kernel(const wind_network *net, const wind_options &o) { return wind_queue_kernel(net, o.mode, o.kernel); }
+
+// Batches at least this large start in the order of an initial-state work
+// estimate when the caller gives no hints (fast mode, compiled-in
+// networks): the persistent fast lane grid holds 2 x 256 x 256 systems at
+// once, and smaller batches start every system immediately.
+constexpr int kAutoOrderMin = 1 << 17;
+
+template <class Net>
+int activity_hints_for(const Context &c, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
+                       unsigned *scratch, hipStream_t st) {
+    typename Net::Params p{};
+    if constexpr (sizeof(typename Net::Params) > 1) std::memcpy(&p, net->params.data(), sizeof(p));
+    return launch_lane_activity<Net>(c.num_cu, p, a, hint, scratch, st);
+}
+
+int activity_hints(const Context &c, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
+                   unsigned *scratch, hipStream_t st) {
+    switch (net->builtin) {
+    case WIND_NET_NR_TEST: return activity_hints_for<NetNRTest>(c, net, a, hint, scratch, st);
+    case WIND_NET_KATZ96: return activity_hints_for<NetKatz96>(c, net, a, hint, scratch, st);
+    case WIND_NET_HHE6: return activity_hints_for<NetHHe6>(c, net, a, hint, scratch, st);
+    default: return fail(WIND_ERR_UNSUPPORTED, "no work estimate for this network");
+    }
+}
 
 template <class K>
 int persistent_grid(const Context &c, K kernel, int block, int nsys, int *grid, const SolverChoice &sc) {
@@ -605,20 +630,6 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     if (rc) return rc;
     WIND_HIP(hipMemsetAsync(c.counters.ptr, 0, 4 * sizeof(unsigned long long), st));
 
-    // Start order from cost hints, for the kernels that take systems from a queue.
-    const int32_t *d_index = nullptr;
-    if (opts.cost_hint && queue_kernel(net, opts)) {
-        const int32_t *d_hint = opts.cost_hint;
-        if (!opts.device_pointers) {
-            rc = reserve(c.hint, (size_t)Nsystems * 4);
-            if (rc) return rc;
-            WIND_HIP(hipMemcpyAsync(c.hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
-            d_hint = (const int32_t *)c.hint.ptr;
-        }
-        rc = order_by_hint(c.order, d_hint, Nsystems, st, &d_index);
-        if (rc) return rc;
-    }
-
     IntegrateArgs a;
     a.tnow = tnow;
     a.tend = tend;
@@ -633,10 +644,35 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     a.max_attempts = opts.max_attempts > 0 ? opts.max_attempts : kDefaultMaxAttempts;
     a.steps_out = d_steps;
     a.status_out = d_status;
-    a.index = d_index;
+    a.index = nullptr;
     a.counters = (unsigned long long *)c.counters.ptr;
 
     WIND_HIP(hipEventRecord(c.ev0, st));
+    // Start order for the kernels that take systems from a queue: from the
+    // caller's cost hints, or (fast mode, compiled-in networks, batches
+    // larger than the device runs at once) from an estimate of each
+    // system's work made from its initial state.
+    if (opts.cost_hint && queue_kernel(net, opts)) {
+        const int32_t *d_hint = opts.cost_hint;
+        if (!opts.device_pointers) {
+            rc = reserve(c.hint, (size_t)Nsystems * 4);
+            if (rc) return rc;
+            WIND_HIP(hipMemcpyAsync(c.hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
+            d_hint = (const int32_t *)c.hint.ptr;
+        }
+        rc = order_by_hint(c.order, d_hint, Nsystems, st, &a.index);
+        if (rc) return rc;
+    } else if (opts.mode == WIND_MODE_FAST && opts.kernel != WIND_KERNEL_BLOCK && Nsystems >= kAutoOrderMin &&
+               (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
+                net->builtin == WIND_NET_HHE6)) {
+        rc = reserve(c.hint, (size_t)Nsystems * 4);
+        if (!rc) rc = reserve(c.activity, ((size_t)Nsystems + 64) * 4);
+        if (rc) return rc;
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)c.activity.ptr, 32);
+        rc = activity_hints(c, net, a, (int32_t *)c.hint.ptr, (unsigned *)c.activity.ptr, st);
+        if (!rc) rc = order_by_hint(c.order, (const int32_t *)c.hint.ptr, Nsystems, st, &a.index);
+        if (rc) return rc;
+    }
     SolverChoice sc;
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;

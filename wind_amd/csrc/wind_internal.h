@@ -80,6 +80,13 @@ template <class Net, int METHOD>
 int launch_lane_fast(int num_cu, const typename Net::Params &p, const wind::IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st);
 
+// Start-order hints of a fast lane launch estimated from the initial state
+// (lane_fast.hip): hint[nsys]; scratch holds 64 + nsys 32-bit words and is
+// cleared by the caller.
+template <class Net>
+int launch_lane_activity(int num_cu, const typename Net::Params &p, const wind::IntegrateArgs &a, int32_t *hint,
+                         unsigned *scratch, hipStream_t st);
+
 // True when launch_mass_action would use the process-wide matrix scratch
 // (matrices too large for LDS): only callers holding the context lock.
 bool mass_action_needs_scratch(const wind_network *net, int mode);
