@@ -218,6 +218,35 @@ def test_batch_scratch_buckets_match_single_calls():
         assert np.array_equal(b["equations"].cpu().numpy(), ya), b["net"].N
 
 
+def test_batch_graph_after_scratch_grows():
+    """A captured batch whose N = 144 exact bucket uses the matrix scratch;
+    a larger single call then grows (reallocates) that scratch; the next
+    graph batch must not replay the old capture with the freed address: its
+    results equal the eager batch's."""
+    torch = torch_cuda()
+    dev = torch.device("cuda:0")
+    N = 144
+    rx, k = W.mass_action_network(N, seed=905)
+    net = api.Network.mass_action(N, rx, k)
+    y0, sc = W.mass_action_systems(8, N, seed=5, scale_range=W.MIXED_SCALE_RANGE)
+    yt = torch.from_numpy(y0).to(dev)
+    ct = torch.from_numpy(sc).to(dev)
+    bk = [dict(net=net, nsys=8, equations=yt.clone(), constants=ct)]
+    kw = dict(mode=api.MODE_EXACT, controller=api.CTRL_REFERENCE, max_attempts=20000)
+    api.integrate_batch(bk, 0.0, 1e2, 1, 1e-3, 1e-3, **kw)
+    torch.cuda.synchronize()
+    eager = bk[0]["equations"].cpu().numpy()
+    for rep in range(2):
+        bk[0]["equations"].copy_(yt)
+        torch.cuda.synchronize()
+        api.integrate_batch(bk, 0.0, 1e2, 1, 1e-3, 1e-3, graph=True, **kw)
+        torch.cuda.synchronize()
+        assert np.array_equal(bk[0]["equations"].cpu().numpy(), eager), rep
+        if rep == 0:
+            yb, cb = W.mass_action_systems(200, N, seed=6, scale_range=W.MIXED_SCALE_RANGE)
+            api.integrate(net, 0.0, 1e1, 1, cb, yb, 200, 1e-3, 1e-3, mode=api.MODE_EXACT, max_attempts=20000)
+
+
 def _requests(seed=3, n=12):
     rng = np.random.default_rng(seed)
     reqs = []

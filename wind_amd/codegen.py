@@ -243,7 +243,7 @@ def _write(path, text):
 
 def build(spec, oracle: bool = True, verbose: bool = False) -> str:
     """Generates and compiles a network; returns the plugin library's path."""
-    from wind_amd.build import ARCH, HIP_FLAGS, HIPCC, LIB
+    from wind_amd.build import ARCH, HIP_FLAGS, HIPCC, shim_location_flags
 
     g = generate(spec)
     name = g["name"]
@@ -268,8 +268,8 @@ def build(spec, oracle: bool = True, verbose: bool = False) -> str:
     for kind, lib in ((1, "sie.so"), (2, "rk2.so"), (3, "sie_host.so")):
         jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-DWIND_SHIM_KIND={kind}",
                      "-DWIND_SHIM_NETWORK=4", f"-DWIND_SHIM_NUM_CONST={g['nconst']}",
-                     os.path.join(CSRC, "legacy_shim.cpp"), "-o", os.path.join(d, lib), f"-L{LIB}", "-lwind_amd",
-                     "-ldl", "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath,$ORIGIN"])
+                     *shim_location_flags("../../libwind_amd.so", os.path.join(d, "libwind_net.so")),
+                     os.path.join(CSRC, "legacy_shim.cpp"), "-o", os.path.join(d, lib), "-ldl"])
     if oracle and os.path.isdir(os.path.join(ROOT, "oracle")):
         osrc = os.path.join(ORACLE_GEN, f"{name}.cpp")
         _write(osrc, g["oracle"])

@@ -179,6 +179,8 @@ int mass_action_scratch_bytes(int num_cu, const wind_network *net, int mode, con
     return WIND_OK;
 }
 
+// Grows the scratch; a growth frees the old buffer, so anything that kept
+// its address (a captured batch graph) must key on mass_action_scratch_ptr().
 int mass_action_scratch_reserve(size_t need) {
     if (need <= g_scratch.bytes) return WIND_OK;
     if (g_scratch.ptr) (void)hipFree(g_scratch.ptr);
@@ -189,6 +191,8 @@ int mass_action_scratch_reserve(size_t need) {
     g_scratch.bytes = need;
     return WIND_OK;
 }
+
+const void *mass_action_scratch_ptr() { return g_scratch.ptr; }
 
 template <int METHOD>
 int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,

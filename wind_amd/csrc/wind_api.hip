@@ -933,6 +933,10 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
             push((uint64_t)__float_as_uint_host(sc[i].share));
         }
         push((uint64_t)(uintptr_t)cnt);
+        // Buckets beyond LDS take the matrix scratch's address as a kernel
+        // argument: a graph captured before the scratch grew (and was freed)
+        // must not be replayed.
+        push((uint64_t)(uintptr_t)mass_action_scratch_ptr());
         if (!c.graph_exec || key != c.graph_key) {
             if (c.graph_exec) {
                 (void)hipGraphExecDestroy(c.graph_exec);

@@ -97,6 +97,8 @@ template <int METHOD>
 int mass_action_scratch_bytes(int num_cu, const wind_network *net, int mode, const SolverChoice &sc, int nsys,
                               size_t *bytes);
 int mass_action_scratch_reserve(size_t bytes);
+// Current address of that scratch (changes when a reserve grows it).
+const void *mass_action_scratch_ptr();
 
 // Enqueues one integration kernel on `st` without synchronising (device
 // pointers in `a`; a.counters must point at 4 zeroed 64-bit counters).
