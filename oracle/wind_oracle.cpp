@@ -470,16 +470,17 @@ int take_step(int method, const oracle_net *net, R tnow, R tend, int nsub, R *y,
 }
 
 template <class R>
-void record(oracle_trace *tr, int n, R tnow, R h, int flag, const R *y1, const R *y2) {
+void record(oracle_trace *tr, int n, R tnow, R h, int flag, int accept, const R *y1, const R *y2) {
     if (!tr) return;
     int a = tr->count++;
     if (a >= tr->capacity) return;
-    tr->tnow[a] = (float)tnow;
-    tr->h[a] = (float)h;
-    tr->flag[a] = flag;
+    if (tr->tnow) tr->tnow[a] = (float)tnow;
+    if (tr->h) tr->h[a] = (float)h;
+    if (tr->flag) tr->flag[a] = flag;
+    if (tr->accept) tr->accept[a] = accept;
     for (int i = 0; i < n; i++) {
-        tr->y1[a * n + i] = (float)y1[i];
-        tr->y2[a * n + i] = (float)y2[i];
+        if (tr->y1) tr->y1[(size_t)a * n + i] = (float)y1[i];
+        if (tr->y2) tr->y2[(size_t)a * n + i] = (float)y2[i];
     }
 }
 
@@ -522,7 +523,7 @@ int integrate_system(int method, const oracle_net *net, R tnow, R tend, R h, R *
                 if (e > err || e != e || err != err) err = (err != err) ? err : e;
             }
             const bool ok = err <= 1.0;
-            record(tr, n, tnow, h, ok ? 0 : 1, y1.data(), y2.data());
+            record(tr, n, tnow, h, ok ? 0 : 1, ok ? 1 : 0, y1.data(), y2.data());
             double fac;
             if (ok)
                 fac = err > 0.0 ? 0.9 / std::sqrt(err) : 4.0;
@@ -544,7 +545,7 @@ int integrate_system(int method, const oracle_net *net, R tnow, R tend, R h, R *
             continue;
         }
         int flag = check_error<R>(method, y1.data(), y2.data(), n, ABS, REL);
-        record(tr, n, tnow, h, flag, y1.data(), y2.data());
+        record(tr, n, tnow, h, flag, (flag && unsolved < 10) ? 0 : 1, y1.data(), y2.data());
         if (flag && unsolved < 10) {
             h = (R)((double)h / 2.0);
             unsolved++;
@@ -713,14 +714,51 @@ int oracle_gold_integrate_mt(int method, const oracle_net *net, float tnow, floa
 int oracle_integrate_system_traced(int method, const oracle_net *net, float tnow, float tend,
                                    float timestep, float *y, const float *c, int N,
                                    float ABSOLUTE, float RELATIVE, oracle_trace *trace,
-                                   long long max_attempts, int32_t *status) {
+                                   long long max_attempts, int32_t *status, int controller) {
     Work<float> w(N);
     if (trace) trace->count = 0;
     int st = 0;
     int n = integrate_system<float>(method, net, tnow, tend, timestep, y, c, N, ABSOLUTE, RELATIVE,
-                                    w, trace, max_attempts, &st);
+                                    w, trace, max_attempts, &st, controller);
     if (status) *status = st;
     return n;
+}
+
+void oracle_integrate_traced_batch(int method, const oracle_net *net, float tnow, float tend, int nfix,
+                                   const float *constants, int const_stride, float *equations, int ns, int n,
+                                   float ABSOLUTE, float RELATIVE, int capacity, float *h, int32_t *accept,
+                                   int32_t *flag, float *y1, float *y2, int32_t *count, int32_t *steps_out,
+                                   int32_t *status_out, long long max_attempts, int controller, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > ns) nthreads = ns > 0 ? ns : 1;
+    auto run = [&](int lo, int hi) {
+        Work<float> w(n);
+        for (int s = lo; s < hi; s++) {
+            const size_t base = (size_t)s * capacity;
+            oracle_trace tr;
+            tr.capacity = capacity;
+            tr.count = 0;
+            tr.tnow = nullptr;
+            tr.h = h ? h + base : nullptr;
+            tr.flag = flag ? flag + base : nullptr;
+            tr.accept = accept ? accept + base : nullptr;
+            tr.y1 = y1 ? y1 + base * n : nullptr;
+            tr.y2 = y2 ? y2 + base * n : nullptr;
+            int st = 0;
+            const int k = integrate_system<float>(method, net, tnow, tend, (tend - tnow) / nfix,
+                                                  equations + (size_t)n * s, constants + (size_t)const_stride * s,
+                                                  n, ABSOLUTE, RELATIVE, w, &tr, max_attempts, &st, controller);
+            if (count) count[s] = tr.count <= capacity ? tr.count : -tr.count;
+            if (steps_out) steps_out[s] = k;
+            if (status_out) status_out[s] = st;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) {
+        const int lo = (int)((long long)ns * t / nthreads), hi = (int)((long long)ns * (t + 1) / nthreads);
+        th.emplace_back(run, lo, hi);
+    }
+    for (auto &x : th) x.join();
 }
 
 void oracle_gje_invert(float *matrix, float *inverse, int N) { gje_invert<float>(matrix, inverse, N); }

@@ -87,6 +87,7 @@ typedef struct oracle_trace {
     int32_t *flag;            /* [capacity] error flag before the cap test */
     float *y1;                /* [capacity*N] */
     float *y2;                /* [capacity*N] */
+    int32_t *accept;          /* [capacity] 1 = the attempt was accepted (optional) */
 } oracle_trace;
 
 /* Restates goldIntegrateSystem (sie_gold.c:81-119 / rk2_gold.c:48-79) with
@@ -118,13 +119,26 @@ int oracle_gold_integrate_f64(int method, const oracle_net *net,
                               int32_t *steps_out, int controller);
 
 /* One system with an attempt trace (restates integrateSystem,
- * common_gold.c:48-153). */
+ * common_gold.c:48-153, and records what its DEBUGBLOCK build prints,
+ * common_gold.c:102-113: y1 and y2 of every attempt). */
 int oracle_integrate_system_traced(int method, const oracle_net *net,
                                    float tnow, float tend, float timestep,
                                    float *y, const float *c, int N,
                                    float ABSOLUTE, float RELATIVE,
                                    oracle_trace *trace, long long max_attempts,
-                                   int32_t *status);
+                                   int32_t *status, int controller);
+
+/* oracle_gold_integrate_ex with every system's attempt trace, laid out as
+ * the library's wind_trace: attempt k of system s at s*capacity + k (h,
+ * accept, flag; y1/y2 at (s*capacity + k)*N), count[s] attempts recorded
+ * (at most capacity; a system with more attempts is integrated to the end
+ * and count[s] = -attempts).  Systems are split over nthreads threads. */
+void oracle_integrate_traced_batch(int method, const oracle_net *net, float tnow, float tend,
+                                   int n_integration_steps, const float *constants, int const_stride,
+                                   float *equations, int Nsystems, int Neqn, float ABSOLUTE, float RELATIVE,
+                                   int capacity, float *h, int32_t *accept, int32_t *flag, float *y1,
+                                   float *y2, int32_t *count, int32_t *steps_out, int32_t *status_out,
+                                   long long max_attempts, int controller, int nthreads);
 
 /* Multi-threaded variant of oracle_gold_integrate for the CPU baseline:
  * systems are split into contiguous slices, one slice per thread. */

@@ -26,7 +26,7 @@ class OracleNet(C.Structure):
 
 class OracleTrace(C.Structure):
     _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("tnow", C.c_void_p), ("h", C.c_void_p),
-                ("flag", C.c_void_p), ("y1", C.c_void_p), ("y2", C.c_void_p)]
+                ("flag", C.c_void_p), ("y1", C.c_void_p), ("y2", C.c_void_p), ("accept", C.c_void_p)]
 
 
 _lib = None
@@ -43,7 +43,10 @@ def lib():
         L.oracle_gold_integrate_ex.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, C.c_longlong, vp, i]
         L.oracle_gold_integrate_mt.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, vp, i, i]
         L.oracle_gold_integrate_f64.argtypes = [i, vp, d, d, i, vp, i, vp, i, i, d, d, vp, i]
-        L.oracle_integrate_system_traced.argtypes = [i, vp, f, f, f, vp, vp, i, f, f, vp, C.c_longlong, vp]
+        L.oracle_integrate_system_traced.argtypes = [i, vp, f, f, f, vp, vp, i, f, f, vp, C.c_longlong, vp, i]
+        L.oracle_integrate_traced_batch.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, i, vp, vp, vp, vp, vp,
+                                                    vp, vp, vp, C.c_longlong, i, i]
+        L.oracle_integrate_traced_batch.restype = None
         L.oracle_gje_invert.argtypes = [vp, vp, i]
         L.oracle_gje_invert.restype = None
         L.oracle_matvec.argtypes = [vp, vp, i]
@@ -153,25 +156,53 @@ def gold_integrate_f64(method, net, tnow, tend, nfix, constants, equations, nsys
     return tot, steps
 
 
-def traced(method, net, tnow, tend, h, y, c, ABS, REL, capacity=100000, max_attempts=1 << 22):
+def traced(method, net, tnow, tend, h, y, c, ABS, REL, capacity=100000, max_attempts=1 << 22, controller=0):
     """One system with its attempt trace; returns (substeps, y_final, trace, status)."""
     L = lib()
     N = net.N
     tr_t = np.zeros(capacity, np.float32)
     tr_h = np.zeros(capacity, np.float32)
     tr_f = np.zeros(capacity, np.int32)
+    tr_a = np.zeros(capacity, np.int32)
     tr_y1 = np.zeros(capacity * N, np.float32)
     tr_y2 = np.zeros(capacity * N, np.float32)
     tr = OracleTrace(capacity, 0, tr_t.ctypes.data, tr_h.ctypes.data, tr_f.ctypes.data,
-                     tr_y1.ctypes.data, tr_y2.ctypes.data)
+                     tr_y1.ctypes.data, tr_y2.ctypes.data, tr_a.ctypes.data)
     y = _f32(y).copy()
     c = _f32(c)
     st = C.c_int32(0)
     n = L.oracle_integrate_system_traced(method, net.ref, tnow, tend, h, y.ctypes.data, c.ctypes.data, N,
-                                         ABS, REL, C.byref(tr), max_attempts, C.byref(st))
+                                         ABS, REL, C.byref(tr), max_attempts, C.byref(st), controller)
     k = min(tr.count, capacity)
-    return n, y, dict(tnow=tr_t[:k], h=tr_h[:k], flag=tr_f[:k], y1=tr_y1[:k * N].reshape(k, N),
+    return n, y, dict(tnow=tr_t[:k], h=tr_h[:k], flag=tr_f[:k], accept=tr_a[:k], y1=tr_y1[:k * N].reshape(k, N),
                       y2=tr_y2[:k * N].reshape(k, N)), st.value
+
+
+def traced_batch(method, net, tnow, tend, nfix, constants, equations, nsys, ABS, REL, capacity, const_stride=None,
+                 max_attempts=0, controller=0, threads=8, with_states=True):
+    """Every system's attempt trace in the library's wind_trace layout.
+    `equations` is updated in place.  Returns a dict with h, accept, flag
+    [nsys, capacity], y1, y2 [nsys, capacity, N] (when with_states), count,
+    steps, status [nsys]; count[s] < 0 marks a system whose -count[s]
+    attempts did not fit in the capacity."""
+    L = lib()
+    N = net.N
+    stride = len(constants) // max(nsys, 1) if const_stride is None else const_stride
+    h = np.zeros((nsys, capacity), np.float32)
+    acc = np.zeros((nsys, capacity), np.int32)
+    flag = np.zeros((nsys, capacity), np.int32)
+    y1 = np.zeros((nsys, capacity, N), np.float32) if with_states else None
+    y2 = np.zeros((nsys, capacity, N), np.float32) if with_states else None
+    count = np.zeros(nsys, np.int32)
+    steps = np.zeros(nsys, np.int32)
+    status = np.zeros(nsys, np.int32)
+    c = _f32(constants)
+    L.oracle_integrate_traced_batch(method, net.ref, tnow, tend, nfix, c.ctypes.data, stride, equations.ctypes.data,
+                                    nsys, N, ABS, REL, capacity, h.ctypes.data, acc.ctypes.data, flag.ctypes.data,
+                                    None if y1 is None else y1.ctypes.data, None if y2 is None else y2.ctypes.data,
+                                    count.ctypes.data, steps.ctypes.data, status.ctypes.data, max_attempts,
+                                    controller, threads)
+    return dict(h=h, accept=acc, flag=flag, y1=y1, y2=y2, count=count, steps=steps, status=status)
 
 
 # ----------------------------------------------------------------- reference

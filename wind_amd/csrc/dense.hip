@@ -11,7 +11,7 @@ using namespace wind;
 template <int METHOD, int PREC>
 int launch_dense(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, int ir, size_t lds, int T,
                  const SolverChoice &sc, hipStream_t st) {
-    auto kfn = ma_dense_integrate<METHOD, PREC>;
+    auto kfn = a.trace.capacity > 0 ? ma_dense_integrate<METHOD, PREC, true> : ma_dense_integrate<METHOD, PREC>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));

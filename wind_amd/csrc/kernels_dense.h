@@ -513,7 +513,9 @@ __device__ inline void dense_refine(const MassActionDevice &net, const float *yv
     __syncthreads();
 }
 
-template <int METHOD, int PREC>
+// TRACE: per-attempt trace and replay through a.trace, as in the lane
+// kernels (kernels_small.h, attempt()).
+template <int METHOD, int PREC, bool TRACE = false>
 __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassActionDevice net, int ir_sweeps) {
     extern __shared__ float lds[];
     const int N = net.N, t = threadIdx.x, T = blockDim.x;
@@ -537,8 +539,11 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
         __syncthreads();
         float tnow = a.tnow, h = a.h0;
         int unsolved = 0, attempts = 0, status = WIND_STATUS_OK;
-        while (tnow < a.tend) {
-            const float hh = fminf(a.tend - tnow, h);
+        const bool replay = TRACE && a.trace.h != nullptr;
+        const int rcount = replay ? a.trace.count[sid] : 0;
+        while (replay ? attempts < rcount : tnow < a.tend) {
+            const size_t slot = TRACE ? (size_t)sid * a.trace.capacity + attempts : 0;
+            const float hh = replay ? a.trace.h[slot] : fminf(a.tend - tnow, h);
             const float tb = tnow + hh;
             const float hs_full = tb - tnow, hs_half = (tb - tnow) / 2.0f;
             for (int pass = 0; pass < 3; pass++) {
@@ -584,10 +589,29 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
                 err = s_err[0];
                 for (int wv = 1; wv < (T + 63) / 64; wv++) err = err_max(err, s_err[wv]);
             }
+            const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
+            if constexpr (TRACE) {
+                if (attempts < a.trace.capacity) {
+                    for (int i = t; i < N; i += T) {
+                        if (a.trace.y1) a.trace.y1[slot * N + i] = L.y1[i];
+                        if (a.trace.y2) a.trace.y2[slot * N + i] = L.yw[i];
+                    }
+                    if (t == 0 && a.trace.flag)
+                        a.trace.flag[slot] = a.controller == WIND_CTRL_REFERENCE ? (int)flag : !(err <= 1.0);
+                }
+                if (replay) {
+                    attempts++;
+                    if (a.trace.accept[slot]) {
+                        tnow = tnow + hh;
+                        for (int i = t; i < N; i += T) L.y[i] = extrap ? 2.0f * L.yw[i] - L.y1[i] : L.yw[i];
+                    }
+                    __syncthreads();
+                    continue;
+                }
+            }
             attempts++;
             const bool acc = control_step<METHOD, true>(a.controller, a.tend, hh, flag, err, tnow, h, unsolved, status);
             if (acc) {
-                const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
                 for (int i = t; i < N; i += T) L.y[i] = extrap ? 2.0f * L.yw[i] - L.y1[i] : L.yw[i];
             }
             __syncthreads();

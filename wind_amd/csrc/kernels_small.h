@@ -221,11 +221,23 @@ struct LaneSys {
 
 // One step-doubling attempt (common_gold.c:77-148) followed by the timestep
 // control of a.controller.  Returns true when the system is finished.
-template <class Net, int METHOD, class M>
+// TRACE: the attempt is recorded in a.trace (y1, y2 and the kernel's own
+// error test, as the reference prints them under DEBUGBLOCK,
+// cuda/common/kernel.cu:214-249, common_gold.c:102-113) and, with a.trace.h
+// set, takes its step and acceptance from the given sequence instead of the
+// controller (replay).
+template <class Net, int METHOD, class M, bool TRACE = false>
 __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArgs &a, const float *c,
-                               LaneSys<Net::N> &s) {
+                               LaneSys<Net::N> &s, int sid = 0) {
     constexpr int N = Net::N;
-    const float hh = fminf(a.tend - s.tnow, s.h);
+    size_t slot = 0;
+    bool replay = false;
+    if constexpr (TRACE) {
+        replay = a.trace.h != nullptr;
+        if (replay && s.attempts >= a.trace.count[sid]) return true;
+        slot = (size_t)sid * a.trace.capacity + s.attempts;
+    }
+    const float hh = replay ? a.trace.h[slot] : fminf(a.tend - s.tnow, s.h);
     const float tb = s.tnow + hh;
     const float hs1 = tb - s.tnow;
     const float hs2 = (tb - s.tnow) / 2.0f;
@@ -243,11 +255,31 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
 #pragma unroll
         for (int i = 0; i < N; i++) err = err_max(err, err_term(y1[i], y2[i], a.ABS, a.REL));
     }
+    const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
+    if constexpr (TRACE) {
+        if (s.attempts < a.trace.capacity) {
+            if (a.trace.y1)
+#pragma unroll
+                for (int i = 0; i < N; i++) a.trace.y1[slot * N + i] = y1[i];
+            if (a.trace.y2)
+#pragma unroll
+                for (int i = 0; i < N; i++) a.trace.y2[slot * N + i] = y2[i];
+            if (a.trace.flag) a.trace.flag[slot] = a.controller == WIND_CTRL_REFERENCE ? (int)flag : !(err <= 1.0);
+        }
+        if (replay) {
+            s.attempts++;
+            if (a.trace.accept[slot]) {
+                s.tnow = s.tnow + hh;
+#pragma unroll
+                for (int i = 0; i < N; i++) s.y[i] = extrap ? 2.0f * y2[i] - y1[i] : y2[i];
+            }
+            return s.attempts >= a.trace.count[sid];
+        }
+    }
     s.attempts++;
     const float hh_used = hh;
     if (control_step<METHOD, M::kFast>(a.controller, a.tend, hh_used, flag, err, s.tnow, s.h, s.unsolved,
                                        s.status)) {
-        const bool extrap = METHOD == WIND_RK2 || a.controller != WIND_CTRL_REFERENCE;
 #pragma unroll
         for (int i = 0; i < N; i++) s.y[i] = extrap ? 2.0f * y2[i] - y1[i] : y2[i];
     }
@@ -311,7 +343,7 @@ __device__ inline void wave_add_steps(unsigned long long *total, unsigned long l
 
 // The body of lane_integrate, shared with kernels that set other launch
 // bounds or are compiled with other floating-point contraction.
-template <class Net, int METHOD, class M>
+template <class Net, int METHOD, class M, bool TRACE = false>
 __device__ inline void lane_integrate_body(const IntegrateArgs &a, const typename Net::Params &p) {
     constexpr int N = Net::N;
     constexpr int NC = Net::NCONST;
@@ -341,7 +373,7 @@ __device__ inline void lane_integrate_body(const IntegrateArgs &a, const typenam
             }
         }
         if (!__any(active)) break;
-        if (active && attempt<Net, METHOD, M>(p, a, c, s)) {
+        if (active && attempt<Net, METHOD, M, TRACE>(p, a, c, s, sid)) {
             float *ys = a.equations + (size_t)sid * N;
             store_state<N>(ys, s.y);
             if (a.steps_out) a.steps_out[sid] = 3 * s.attempts;
@@ -354,9 +386,9 @@ __device__ inline void lane_integrate_body(const IntegrateArgs &a, const typenam
     wave_add_steps(a.counters + 1, steps);
 }
 
-template <class Net, int METHOD, class M>
+template <class Net, int METHOD, class M, bool TRACE = false>
 __global__ void __launch_bounds__(256) lane_integrate(IntegrateArgs a, typename Net::Params p) {
-    lane_integrate_body<Net, METHOD, M>(a, p);
+    lane_integrate_body<Net, METHOD, M, TRACE>(a, p);
 }
 
 // ---------------------------------------------------------------------------

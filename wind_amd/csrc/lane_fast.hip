@@ -23,6 +23,13 @@ __global__ void __launch_bounds__(256, WIND_LANE_FAST_MIN_BLOCKS)
     lane_integrate_body<Net, METHOD, MathFast>(a, p);
 }
 
+// The same integration with a per-attempt trace or replay (a.trace); the
+// arithmetic of each attempt is lane_fast_kernel's.
+template <class Net, int METHOD>
+__global__ void __launch_bounds__(256) lane_fast_trace_kernel(IntegrateArgs a, typename Net::Params p) {
+    lane_integrate_body<Net, METHOD, MathFast, true>(a, p);
+}
+
 // Start-order estimate from the initial state alone: x = (tend - tnow) *
 // max_i |f_i(y0)| / (|y0_i| + ABS), the fastest relative rate of change
 // over the interval, one right-hand side per system.  Its float bits go to
@@ -99,7 +106,7 @@ constexpr int kLaneFastBlocksPerCU = 2;
 template <class Net, int METHOD>
 int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
-    auto k = lane_fast_kernel<Net, METHOD>;
+    auto k = a.trace.capacity > 0 ? lane_fast_trace_kernel<Net, METHOD> : lane_fast_kernel<Net, METHOD>;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));

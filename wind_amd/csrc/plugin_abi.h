@@ -9,7 +9,7 @@
 
 #include "wind_types.h"
 
-#define WIND_PLUGIN_ABI 1
+#define WIND_PLUGIN_ABI 2
 
 struct wind_plugin_desc {
     int abi;          // WIND_PLUGIN_ABI

@@ -71,6 +71,7 @@ int plugin_launch_fast_method(const IntegrateArgs &a, int num_cu, float share, i
 template <class Net>
 int plugin_launch(const IntegrateArgs *a, int method, int mode, int kernel, int num_cu, float share, int grid,
                   hipStream_t st) {
+    if (a->trace.capacity > 0) return WIND_ERR_UNSUPPORTED;  // no traced kernels in plugins
     if (method == WIND_SIE) return plugin_launch_method<Net, WIND_SIE>(*a, mode, kernel, num_cu, share, grid, st);
     if (method == WIND_RK2) return plugin_launch_method<Net, WIND_RK2>(*a, mode, kernel, num_cu, share, grid, st);
     return WIND_ERR_ARGUMENT;

@@ -6,6 +6,22 @@
 
 namespace wind {
 
+// Per-attempt trace of an integration (wind_trace in wind.h, device
+// pointers).  capacity == 0: no trace.  With h set, attempt k of system s
+// takes the step h[s*capacity + k] and its acceptance from accept[...],
+// and the system stops after count[s] attempts (replay); y1, y2 and flag,
+// when set, receive each attempt's full-step and two-half-step states and
+// the kernel's own error test (1 = its controller would reject).
+struct TraceArgs {
+    int32_t capacity = 0;
+    const float *h = nullptr;
+    const int32_t *accept = nullptr;
+    const int32_t *count = nullptr;
+    float *y1 = nullptr;
+    float *y2 = nullptr;
+    int32_t *flag = nullptr;
+};
+
 // Kernel arguments common to every integrator kernel.  Pointers are device
 // pointers.  Systems are stored as in the reference ABI: equations[s*N + i],
 // constants[s*const_stride + k].
@@ -22,6 +38,7 @@ struct IntegrateArgs {
     int32_t *status_out;           // optional [nsys]
     const int32_t *index;          // optional [nsys]: permutation of system ids
     unsigned long long *counters;  // [0] work-queue head, [1] total sub-steps
+    TraceArgs trace;               // per-attempt trace / replay (kernels that support it)
 };
 
 }  // namespace wind
