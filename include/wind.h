@@ -189,6 +189,27 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
                    const float *constants, float *equations, int Nsystems, int Neqn,
                    float ABSOLUTE, float RELATIVE, const wind_options *opts);
 
+/* Allocation-free variant of wind_integrate.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, semantics and results as wind_integrate, with
+ * these limits:
+ *   - opts->device_pointers must be 1 (constants, equations, steps_out,
+ *     status_out and cost_hint are device pointers; the caller owns them);
+ *   - Nsystems <= WIND_NOALLOC_MAX_SYSTEMS, else WIND_ERR_ARGUMENT;
+ *   - opts->kernel_ms must be NULL (WIND_ERR_ARGUMENT);
+ *   - mass-action networks whose matrices do not fit in LDS (exact mode,
+ *     large N) return WIND_ERR_UNSUPPORTED.
+ * It calls no hipMalloc/hipFree: the work-queue counters and start-order
+ * buffers live in one static device array of about 12.6 MB per GPU, held
+ * until the process ends, which every call on that GPU shares.  Calls within
+ * a process are serialised; the call returns after the stream (opts->stream,
+ * NULL = the null stream) has finished the integration. */
+#define WIND_NOALLOC_MAX_SYSTEMS (1 << 20)
+int wind_integrate_noalloc(const wind_network *net, float tnow, float tend, int n_integration_steps,
+                           const float *constants, float *equations, int Nsystems, int Neqn,
+                           float ABSOLUTE, float RELATIVE, const wind_options *opts);
+
 /* Output-step driver: the reference harness's outer loop
  * (python/ode_systems/ode_base.py:315-397, runIntegratorOutput) with the
  * state resident on the device between output steps.  Integrates over

@@ -116,3 +116,36 @@ def test_options_struct_matches_header():
     kinds = {C.c_void_p: "ptr", C.c_int64: "int64", C.c_int: "int"}
     fields = [(n, kinds[t]) for n, t in api.WindOptions._fields_]
     assert fields == members
+
+
+def test_noalloc_argument_errors_before_device():
+    """wind_integrate_noalloc rejects what would need an allocation before
+    any device call: host pointers, kernel timing, more systems than its
+    static scratch holds."""
+    lib = api.load()
+    net = api.Network.builtin(api.NET_NR_TEST)
+    o = api.WindOptions()
+    lib.wind_options_init(C.byref(o))
+    y = np.zeros(4, np.float32)
+    c = np.zeros(8, np.float32)
+    rc = lib.wind_integrate_noalloc(net.handle, 0.0, 1.0, 1, c.ctypes.data, y.ctypes.data, 2, 2, 1e-3, 1e-3,
+                                    C.byref(o))
+    assert rc == -1 and b"device pointers" in lib.wind_last_error()
+    o.device_pointers = 1
+    kms = C.c_float(0.0)
+    o.kernel_ms = C.cast(C.byref(kms), C.c_void_p)
+    rc = lib.wind_integrate_noalloc(net.handle, 0.0, 1.0, 1, c.ctypes.data, y.ctypes.data, 2, 2, 1e-3, 1e-3,
+                                    C.byref(o))
+    assert rc == -1 and b"kernel_ms" in lib.wind_last_error()
+    o.kernel_ms = None
+    text = open(os.path.join(ROOT, "include", "wind.h")).read()
+    limit = eval(re.search(r"#define WIND_NOALLOC_MAX_SYSTEMS \((.*?)\)", text).group(1))
+    rc = lib.wind_integrate_noalloc(net.handle, 0.0, 1.0, 1, c.ctypes.data, y.ctypes.data, limit + 1, 2, 1e-3,
+                                    1e-3, C.byref(o))
+    assert rc == -1 and b"exceeds its scratch maximum" in lib.wind_last_error()
+    rc = lib.wind_integrate_noalloc(net.handle, 0.0, 1.0, 1, c.ctypes.data, y.ctypes.data, 2, 3, 1e-3, 1e-3,
+                                    C.byref(o))
+    assert rc == -1 and b"does not match the network" in lib.wind_last_error()
+    # nothing to integrate: no device call
+    assert lib.wind_integrate_noalloc(net.handle, 1.0, 1.0, 1, c.ctypes.data, y.ctypes.data, 2, 2, 1e-3, 1e-3,
+                                      C.byref(o)) == 0

@@ -27,7 +27,7 @@ STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 EXPORTED = (
     "wind_network_create_builtin", "wind_network_create_mass_action", "wind_network_destroy",
     "wind_network_size", "wind_network_nconst", "wind_network_create_plugin", "wind_options_init",
-    "wind_integrate", "wind_integrate_lockstep", "wind_integrate_outputs",
+    "wind_integrate", "wind_integrate_noalloc", "wind_integrate_lockstep", "wind_integrate_outputs",
     "wind_last_error", "wind_version", "wind_integrate_batch", "wind_server_create", "wind_server_destroy",
     "wind_server_block_create", "wind_server_block_read", "wind_server_block_destroy", "wind_server_submit",
     "wind_server_wait", "wind_server_inflight",
@@ -90,6 +90,7 @@ def load(path: str = LIB_PATH):
     lib.wind_options_init.restype = None
     lib.wind_integrate.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(WindOptions)]
+    lib.wind_integrate_noalloc.argtypes = lib.wind_integrate.argtypes
     lib.wind_integrate_outputs.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p,
                                            C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float,
                                            C.POINTER(WindOptions), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]

@@ -201,64 +201,64 @@ bool queue_kernel(const wind_network *net, const wind_options &o) { return wind_
 constexpr int kAutoOrderMin = 1 << 17;
 
 template <class Net>
-int activity_hints_for(const Context &c, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
+int activity_hints_for(int num_cu, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
                        unsigned *scratch, hipStream_t st) {
     typename Net::Params p{};
     if constexpr (sizeof(typename Net::Params) > 1) std::memcpy(&p, net->params.data(), sizeof(p));
-    return launch_lane_activity<Net>(c.num_cu, p, a, hint, scratch, st);
+    return launch_lane_activity<Net>(num_cu, p, a, hint, scratch, st);
 }
 
-int activity_hints(const Context &c, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
+int activity_hints(int num_cu, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
                    unsigned *scratch, hipStream_t st) {
     switch (net->builtin) {
-    case WIND_NET_NR_TEST: return activity_hints_for<NetNRTest>(c, net, a, hint, scratch, st);
-    case WIND_NET_KATZ96: return activity_hints_for<NetKatz96>(c, net, a, hint, scratch, st);
-    case WIND_NET_HHE6: return activity_hints_for<NetHHe6>(c, net, a, hint, scratch, st);
+    case WIND_NET_NR_TEST: return activity_hints_for<NetNRTest>(num_cu, net, a, hint, scratch, st);
+    case WIND_NET_KATZ96: return activity_hints_for<NetKatz96>(num_cu, net, a, hint, scratch, st);
+    case WIND_NET_HHE6: return activity_hints_for<NetHHe6>(num_cu, net, a, hint, scratch, st);
     default: return fail(WIND_ERR_UNSUPPORTED, "no work estimate for this network");
     }
 }
 
 template <class K>
-int persistent_grid(const Context &c, K kernel, int block, int nsys, int *grid, const SolverChoice &sc) {
+int persistent_grid(int num_cu, K kernel, int block, int nsys, int *grid, const SolverChoice &sc) {
     int per_cu = 0;
     WIND_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
-    *grid = scaled_grid(c.num_cu, per_cu, sc, nsys, block);
+    *grid = scaled_grid(num_cu, per_cu, sc, nsys, block);
     return WIND_OK;
 }
 
 template <class Net, int METHOD>
-int launch_small(const Context &c, const wind_network *net, const IntegrateArgs &a, int mode,
+int launch_small(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
                  int kernel, const SolverChoice &sc, hipStream_t st) {
     typename Net::Params p{};
     if constexpr (sizeof(typename Net::Params) > 1) std::memcpy(&p, net->params.data(), sizeof(p));
     if (kernel == WIND_KERNEL_BLOCK) {
         if (mode != WIND_MODE_EXACT)
             return fail(WIND_ERR_UNSUPPORTED, "the block kernel implements exact mode only");
-        const int grid = scaled_grid(c.num_cu, 32, sc, a.nsys);
+        const int grid = scaled_grid(num_cu, 32, sc, a.nsys);
         hipLaunchKernelGGL((block_integrate<Net, METHOD>), dim3(grid), dim3(64), 0, st, a, p);
     } else if (mode == WIND_MODE_EXACT) {
         auto k = lane_integrate<Net, METHOD, MathExact>;
         int grid = 0;
-        int rc = persistent_grid(c, k, 256, a.nsys, &grid, sc);
+        int rc = persistent_grid(num_cu, k, 256, a.nsys, &grid, sc);
         if (rc) return rc;
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     } else {
-        return launch_lane_fast<Net, METHOD>(c.num_cu, p, a, sc, st);
+        return launch_lane_fast<Net, METHOD>(num_cu, p, a, sc, st);
     }
     WIND_HIP(hipGetLastError());
     return WIND_OK;
 }
 
 template <int METHOD>
-int dispatch(const Context &c, const wind_network *net, const IntegrateArgs &a, int mode,
+int dispatch(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
              int kernel, const SolverChoice &sc, hipStream_t st) {
     switch (net->builtin) {
-    case WIND_NET_NR_TEST: return launch_small<NetNRTest, METHOD>(c, net, a, mode, kernel, sc, st);
-    case WIND_NET_KATZ96: return launch_small<NetKatz96, METHOD>(c, net, a, mode, kernel, sc, st);
-    case WIND_NET_HHE6: return launch_small<NetHHe6, METHOD>(c, net, a, mode, kernel, sc, st);
-    case WIND_NET_MASS_ACTION: return launch_mass_action<METHOD>(c.num_cu, net, a, mode, kernel, sc, st);
+    case WIND_NET_NR_TEST: return launch_small<NetNRTest, METHOD>(num_cu, net, a, mode, kernel, sc, st);
+    case WIND_NET_KATZ96: return launch_small<NetKatz96, METHOD>(num_cu, net, a, mode, kernel, sc, st);
+    case WIND_NET_HHE6: return launch_small<NetHHe6, METHOD>(num_cu, net, a, mode, kernel, sc, st);
+    case WIND_NET_MASS_ACTION: return launch_mass_action<METHOD>(num_cu, net, a, mode, kernel, sc, st);
     case WIND_NET_PLUGIN: {
-        const int rc = net->plugin->launch(&a, METHOD, mode, kernel, c.num_cu, sc.share, sc.grid, st);
+        const int rc = net->plugin->launch(&a, METHOD, mode, kernel, num_cu, sc.share, sc.grid, st);
         if (rc == WIND_ERR_UNSUPPORTED) return fail(rc, "generated networks: the block kernel is exact mode only");
         return rc ? fail(rc, "generated network: kernel launch failed") : WIND_OK;
     }
@@ -269,6 +269,35 @@ int dispatch(const Context &c, const wind_network *net, const IntegrateArgs &a, 
 }  // namespace
 
 int wind_fail(int code, const char *msg) { return fail(code, msg); }
+
+// Argument checks shared by wind_integrate and wind_integrate_noalloc; also
+// clears the optional outputs.
+static int check_integrate_args(const wind_network *net, const wind_options &opts, int Nsystems, int Neqn,
+                                int n_integration_steps) {
+    if (opts.total_steps) *opts.total_steps = 0;
+    if (opts.kernel_ms) *opts.kernel_ms = 0.0f;
+    if (!net) return fail(WIND_ERR_ARGUMENT, "network is NULL");
+    if (opts.method != WIND_SIE && opts.method != WIND_RK2)
+        return fail(WIND_ERR_ARGUMENT, "method must be WIND_SIE or WIND_RK2");
+    if (opts.mode != WIND_MODE_EXACT && opts.mode != WIND_MODE_FAST)
+        return fail(WIND_ERR_ARGUMENT, "mode must be WIND_MODE_EXACT or WIND_MODE_FAST");
+    if (opts.controller != WIND_CTRL_REFERENCE && opts.controller != WIND_CTRL_ADAPTIVE)
+        return fail(WIND_ERR_ARGUMENT, "controller must be WIND_CTRL_REFERENCE or WIND_CTRL_ADAPTIVE");
+    if (opts.solver < WIND_SOLVER_AUTO || opts.solver > WIND_SOLVER_MFMA_BF16X3)
+        return fail(WIND_ERR_ARGUMENT, "unknown solver");
+    if (opts.ir_sweeps < 0 || opts.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
+    if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
+    if (Neqn != net->N)
+        return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network (N = " +
+                                           std::to_string(net->N) + ")");
+    if (n_integration_steps < 0)
+        return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0 (negative initial step)");
+    return WIND_OK;
+}
+
+// Whether a launch of this size starts in the order of an initial-state
+// work estimate (no caller hints; fast lane kernels of compiled-in networks).
+static bool auto_order(const wind_network *net, const wind_options &opts, int Nsystems);
 
 static inline uint32_t __float_as_uint_host(float f) {
     uint32_t u;
@@ -526,8 +555,8 @@ int wind_integrate_outputs(const wind_network *net, double tnow, double tend, in
             a.status_out = nullptr;
             a.index = d_index;
             a.counters = cnt + 4 * (k - 1);
-            rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, opts.mode, opts.kernel, sc, st)
-                                         : dispatch<WIND_RK2>(c, net, a, opts.mode, opts.kernel, sc, st);
+            rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c.num_cu, net, a, opts.mode, opts.kernel, sc, st)
+                                         : dispatch<WIND_RK2>(c.num_cu, net, a, opts.mode, opts.kernel, sc, st);
             if (rc) return rc;
             have_steps = ordered;
         }
@@ -572,24 +601,8 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
         opts = *opts_in;
     else
         wind_options_init(&opts);
-    if (opts.total_steps) *opts.total_steps = 0;
-    if (opts.kernel_ms) *opts.kernel_ms = 0.0f;
-    if (!net) return fail(WIND_ERR_ARGUMENT, "network is NULL");
-    if (opts.method != WIND_SIE && opts.method != WIND_RK2)
-        return fail(WIND_ERR_ARGUMENT, "method must be WIND_SIE or WIND_RK2");
-    if (opts.mode != WIND_MODE_EXACT && opts.mode != WIND_MODE_FAST)
-        return fail(WIND_ERR_ARGUMENT, "mode must be WIND_MODE_EXACT or WIND_MODE_FAST");
-    if (opts.controller != WIND_CTRL_REFERENCE && opts.controller != WIND_CTRL_ADAPTIVE)
-        return fail(WIND_ERR_ARGUMENT, "controller must be WIND_CTRL_REFERENCE or WIND_CTRL_ADAPTIVE");
-    if (opts.solver < WIND_SOLVER_AUTO || opts.solver > WIND_SOLVER_MFMA_BF16X3)
-        return fail(WIND_ERR_ARGUMENT, "unknown solver");
-    if (opts.ir_sweeps < 0 || opts.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
-    if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
-    if (Neqn != net->N)
-        return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network (N = " +
-                                           std::to_string(net->N) + ")");
-    if (n_integration_steps < 0)
-        return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0 (negative initial step)");
+    int rc = check_integrate_args(net, opts, Nsystems, Neqn, n_integration_steps);
+    if (rc) return rc;
     // As in the reference, systems may share constants when the stride is
     // below the network's constant count (cuda/include/ode_gold.h:1 ships
     // NUM_CONST = 2 for a network that reads 10); the caller's array must
@@ -600,7 +613,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
 
     Context &c = ctx();
     std::lock_guard<std::mutex> lock(c.mu);
-    int rc = ensure_ready(c);
+    rc = ensure_ready(c);
     if (rc) return rc;
     hipStream_t st = opts.stream ? (hipStream_t)opts.stream : c.stream;
 
@@ -662,22 +675,20 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
         }
         rc = order_by_hint(c.order, d_hint, Nsystems, st, &a.index);
         if (rc) return rc;
-    } else if (opts.mode == WIND_MODE_FAST && opts.kernel != WIND_KERNEL_BLOCK && Nsystems >= kAutoOrderMin &&
-               (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
-                net->builtin == WIND_NET_HHE6)) {
+    } else if (auto_order(net, opts, Nsystems)) {
         rc = reserve(c.hint, (size_t)Nsystems * 4);
         if (!rc) rc = reserve(c.activity, ((size_t)Nsystems + 64) * 4);
         if (rc) return rc;
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)c.activity.ptr, 32);
-        rc = activity_hints(c, net, a, (int32_t *)c.hint.ptr, (unsigned *)c.activity.ptr, st);
+        rc = activity_hints(c.num_cu, net, a, (int32_t *)c.hint.ptr, (unsigned *)c.activity.ptr, st);
         if (!rc) rc = order_by_hint(c.order, (const int32_t *)c.hint.ptr, Nsystems, st, &a.index);
         if (rc) return rc;
     }
     SolverChoice sc;
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
-    rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, opts.mode, opts.kernel, sc, st)
-                                 : dispatch<WIND_RK2>(c, net, a, opts.mode, opts.kernel, sc, st);
+    rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c.num_cu, net, a, opts.mode, opts.kernel, sc, st)
+                                 : dispatch<WIND_RK2>(c.num_cu, net, a, opts.mode, opts.kernel, sc, st);
     if (rc) return rc;
     WIND_HIP(hipEventRecord(c.ev1, st));
 
@@ -700,13 +711,150 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
 
 }  // extern "C"
 
+static bool auto_order(const wind_network *net, const wind_options &opts, int Nsystems) {
+    return opts.mode == WIND_MODE_FAST && opts.kernel != WIND_KERNEL_BLOCK && Nsystems >= kAutoOrderMin &&
+           (net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 || net->builtin == WIND_NET_HHE6);
+}
+
+// ---------------------------------------------------------------------------
+// wind_integrate_noalloc: wind_integrate for callers that cannot allocate.
+// Everything the kernels need beyond the caller's inputs and outputs (the
+// work-queue head and sub-step total, and the start-order buffers) lives in
+// one statically sized device array, `scratch`.  Its device address is taken
+// once per device with hipGetSymbolAddress (the array's name is only a
+// host-side stand-in) and used for every kernel argument.  The array is
+// shared by every call on a device and keeps the last call's contents; the
+// calls of one process are serialised by g_noalloc_mu, so they never overlap
+// on it.
+// ---------------------------------------------------------------------------
+namespace {
+
+// 32-bit words of the scratch layout.
+constexpr int kNoAllocCounterWords = 64;               // 4 x u64 counters, padded to 256 B
+constexpr int kNoAllocCountWords = kHintCountWords;     // start-order class counts
+constexpr size_t kNoAllocWords = (size_t)kNoAllocCounterWords + kNoAllocCountWords +
+                                 (64 + (size_t)WIND_NOALLOC_MAX_SYSTEMS) +  // activity estimates
+                                 (size_t)WIND_NOALLOC_MAX_SYSTEMS +         // hints
+                                 (size_t)WIND_NOALLOC_MAX_SYSTEMS;          // start order
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+struct NoAllocDevice {
+    bool ready = false;
+    uint32_t *scratch = nullptr;  // device address of `scratch` on this device
+    int num_cu = 0;
+};
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kMaxDevices];
+
+// Once per device, with that device current: the array's device address
+// and the CU count.
+int noalloc_setup(int dev, NoAllocDevice &d) {
+    if (d.ready) return WIND_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(WIND_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    WIND_HIP(hipDeviceGetAttribute(&d.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    d.scratch = (uint32_t *)ptr;
+    d.ready = true;
+    return WIND_OK;
+}
+
+}  // namespace
+
+extern "C" int wind_integrate_noalloc(const wind_network *net, float tnow, float tend, int n_integration_steps,
+                                      const float *constants, float *equations, int Nsystems, int Neqn,
+                                      float ABSOLUTE, float RELATIVE, const wind_options *opts_in) {
+    g_last_error.clear();
+    wind_options opts;
+    if (opts_in)
+        opts = *opts_in;
+    else
+        wind_options_init(&opts);
+    int rc = check_integrate_args(net, opts, Nsystems, Neqn, n_integration_steps);
+    if (rc) return rc;
+    if (!opts.device_pointers)
+        return fail(WIND_ERR_ARGUMENT, "wind_integrate_noalloc takes device pointers only (device_pointers = 1)");
+    if (opts.kernel_ms) return fail(WIND_ERR_ARGUMENT, "wind_integrate_noalloc does not time its kernels (kernel_ms)");
+    if (Nsystems > WIND_NOALLOC_MAX_SYSTEMS)
+        return fail(WIND_ERR_ARGUMENT, "wind_integrate_noalloc: Nsystems " + std::to_string(Nsystems) +
+                                           " exceeds its scratch maximum of " +
+                                           std::to_string(WIND_NOALLOC_MAX_SYSTEMS) + " systems");
+    if (net->builtin == WIND_NET_MASS_ACTION && mass_action_needs_scratch(net, opts.mode))
+        return fail(WIND_ERR_UNSUPPORTED,
+                    "wind_integrate_noalloc: this network's matrices do not fit in LDS (they need an allocation)");
+    const int stride = opts.const_stride > 0 ? opts.const_stride : net->nconst;
+    if (Nsystems == 0 || !(tnow < tend)) return WIND_OK;
+    if (!constants || !equations) return fail(WIND_ERR_ARGUMENT, "NULL state or constants");
+
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    int dev = 0;
+    WIND_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(WIND_ERR_DEVICE, "device index out of range");
+    NoAllocDevice &d = g_noalloc[dev];
+    rc = noalloc_setup(dev, d);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)opts.stream;  // NULL: the device's null stream
+    uint32_t *base = d.scratch;
+    unsigned long long *counters = (unsigned long long *)base;
+    unsigned *counts = base + kNoAllocCounterWords;
+    unsigned *activity = counts + kNoAllocCountWords;
+    int32_t *hint = (int32_t *)(activity + 64 + WIND_NOALLOC_MAX_SYSTEMS);
+    int32_t *order = hint + WIND_NOALLOC_MAX_SYSTEMS;
+    // The counters start at zero (a fill kernel, not a memset).
+    wind_zero_counters(counters, 4, st);
+
+    IntegrateArgs a;
+    a.tnow = tnow;
+    a.tend = tend;
+    a.h0 = (tend - tnow) / (float)n_integration_steps;
+    a.ABS = ABSOLUTE;
+    a.REL = RELATIVE;
+    a.controller = opts.controller;
+    a.constants = constants;
+    a.equations = equations;
+    a.nsys = Nsystems;
+    a.const_stride = stride;
+    a.max_attempts = opts.max_attempts > 0 ? opts.max_attempts : kDefaultMaxAttempts;
+    a.steps_out = opts.steps_out;
+    a.status_out = opts.status_out;
+    a.index = nullptr;
+    a.counters = counters;
+    // The same start orders as wind_integrate, built in the scratch array.
+    if (opts.cost_hint && queue_kernel(net, opts)) {
+        wind_order_by_hint(opts.cost_hint, Nsystems, order, counts, st);
+        WIND_HIP(hipGetLastError());
+        a.index = order;
+    } else if (auto_order(net, opts, Nsystems)) {
+        wind_zero_counters((unsigned long long *)activity, 32, st);
+        rc = activity_hints(d.num_cu, net, a, hint, activity, st);
+        if (rc) return rc;
+        wind_order_by_hint(hint, Nsystems, order, counts, st);
+        WIND_HIP(hipGetLastError());
+        a.index = order;
+    }
+    SolverChoice sc;
+    sc.solver = opts.solver;
+    sc.ir_sweeps = opts.ir_sweeps;
+    rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(d.num_cu, net, a, opts.mode, opts.kernel, sc, st)
+                                 : dispatch<WIND_RK2>(d.num_cu, net, a, opts.mode, opts.kernel, sc, st);
+    if (rc) return rc;
+    unsigned long long total = 0;
+    WIND_HIP(hipMemcpyAsync(&total, counters + 1, sizeof(total), hipMemcpyDeviceToHost, st));
+    WIND_HIP(hipStreamSynchronize(st));
+    if (opts.total_steps) *opts.total_steps = (int64_t)total;
+    return WIND_OK;
+}
+
 namespace {
 
 
 int launch_bucket(const Context &c, const wind_network *net, const IntegrateArgs &a, const wind_options &o,
                   const SolverChoice &sc, hipStream_t st) {
-    return o.method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, o.mode, WIND_KERNEL_AUTO, sc, st)
-                                : dispatch<WIND_RK2>(c, net, a, o.mode, WIND_KERNEL_AUTO, sc, st);
+    return o.method == WIND_SIE ? dispatch<WIND_SIE>(c.num_cu, net, a, o.mode, WIND_KERNEL_AUTO, sc, st)
+                                : dispatch<WIND_RK2>(c.num_cu, net, a, o.mode, WIND_KERNEL_AUTO, sc, st);
 }
 
 IntegrateArgs bucket_args(const wind_bucket &b, float tnow, float tend, int nfix, float ABS, float REL,
@@ -999,8 +1147,8 @@ int wind_launch_async(const wind_network *net, const IntegrateArgs &a, int metho
     }
     if (net->builtin == WIND_NET_MASS_ACTION && mass_action_needs_scratch(net, mode))
         return fail(WIND_ERR_UNSUPPORTED, "this network's matrices do not fit in LDS (it needs the shared scratch buffer)");
-    return method == WIND_SIE ? dispatch<WIND_SIE>(c, net, a, mode, kernel, sc, st)
-                              : dispatch<WIND_RK2>(c, net, a, mode, kernel, sc, st);
+    return method == WIND_SIE ? dispatch<WIND_SIE>(c.num_cu, net, a, mode, kernel, sc, st)
+                              : dispatch<WIND_RK2>(c.num_cu, net, a, mode, kernel, sc, st);
 }
 
 int wind_device_cus(int *num_cu) {
