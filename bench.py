@@ -51,35 +51,79 @@ SEED = 1
 TOL = 2e-6
 SWEEP_TOLS = (1e-5, 5e-6)  # looser tolerances reported beside the headline
 REF_TOL = 1e-3      # the reference controller's run
-# Algorithmic work per sub-step of hhe6 (counted once, LAPACK-style):
-#   network RHS + Jacobian 674 flops (+ 8 exp, 1 log, 3 div, counted apart),
-#   form I - hJ 2N^2, LU 2/3 N^3, two triangular solves 2N^2, update 2N.
+# Algorithmic work of hhe6 (counted once, LAPACK-style): network RHS +
+# Jacobian 674 flops (+ 8 exp, 1 log, 3 div, counted apart); per solve: form
+# I - hJ 2N^2, LU 2/3 N^3, two triangular solves 2N^2, update 2N.  An attempt
+# evaluates the network twice (at y for the full step and the first half
+# step, and at the half-step state) and solves three systems; per sub-step
+# (3 per attempt) that is F_SUBSTEP.  The executed count of the fast kernel
+# comes from the FP32 instruction counters in profiles/valu_counts.json.
 F_NET = 674
-F_SUBSTEP = F_NET + 2 * N_SPECIES**2 + (2 * N_SPECIES**3) // 3 + 2 * N_SPECIES**2 + 2 * N_SPECIES
+F_SOLVE = 2 * N_SPECIES**2 + (2 * N_SPECIES**3) // 3 + 2 * N_SPECIES**2 + 2 * N_SPECIES
+F_SUBSTEP = (2 * F_NET + 3 * F_SOLVE) / 3.0
 # Algorithmic HBM bytes per system: state in + out (2N floats) + 1 constant.
 BYTES_PER_SYSTEM = 4 * (2 * N_SPECIES + 1)
 PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # published FP32 vector peak (counts packed dual-issue FP32)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # one FMA per lane per cycle: the ceiling without packed FP32
+
+
+def host_cores():
+    """Threads for the CPU baseline: the CPUs this process may run on
+    (sched_getaffinity), capped by the cgroup CPU quota when one is set.
+    Returns (threads, visible CPUs, quota or None)."""
+    vis = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    n = vis if quota is None else max(1, min(vis, int(quota + 0.5)))
+    return n, vis, quota
 
 
 def rel_err(y, truth, floor=1e-3):
     return np.max(np.abs(y - truth) / np.maximum(np.abs(truth), floor), axis=1)
 
 
-def accuracy(yf, nsys):
-    tp = os.path.join(ROOT, "tests", "golden", "truth_hhe6.npz")
-    if not os.path.exists(tp):
-        return None
-    T = np.load(tp)
-    idx = T["idx"]
-    keep = idx < nsys
-    if not keep.any():
-        return None
-    e = rel_err(yf[idx[keep]].astype(np.float64), T["y"][keep])
-    e = np.where(np.isnan(e), np.inf, e)
-    return {"sample": int(keep.sum()), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
+def _acc_stats(e):
+    return {"sample": int(len(e)), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
             "median_rel_err": float(np.median(e)), "p90_rel_err": float(np.quantile(e, 0.9)),
-            "max_rel_err": float(np.max(e)), "nonfinite": int(np.sum(~np.isfinite(e)))}
+            "p99_rel_err": float(np.quantile(e, 0.99)), "max_rel_err": float(np.max(e)),
+            "nonfinite": int(np.sum(~np.isfinite(e)))}
+
+
+def accuracy(yf, nsys):
+    """Final-state error against FP64 Radau solutions: the 512-system
+    uniform sample (every 2048th system, truth_hhe6.npz) and the 8192-system
+    tail sample (6144 stratified over log T x log n_H plus the 2048 with the
+    most sub-steps, truth_hhe6_tail.npz); the headline figures are over both."""
+    parts, out = [], {}
+    for name, fn in (("uniform", "truth_hhe6.npz"), ("tail", "truth_hhe6_tail.npz")):
+        tp = os.path.join(ROOT, "tests", "golden", fn)
+        if not os.path.exists(tp):
+            continue
+        T = np.load(tp)
+        idx = T["idx"]
+        keep = idx < nsys
+        if not keep.any():
+            continue
+        e = rel_err(yf[idx[keep]].astype(np.float64), T["y"][keep])
+        e = np.where(np.isnan(e), np.inf, e)
+        parts.append(e)
+        out[name] = _acc_stats(e)
+        if name == "tail":
+            ns = int(T["n_stratified"])
+            out[name]["stratified"] = _acc_stats(e[:ns])
+            out[name]["most_substeps"] = _acc_stats(e[ns:])
+    if not parts:
+        return None
+    res = _acc_stats(np.concatenate(parts))
+    res.update(out)
+    return res
 
 
 def cpu_baseline(params, y0, nH, seconds, tol, controller):
@@ -90,7 +134,7 @@ def cpu_baseline(params, y0, nH, seconds, tol, controller):
     import oracle_lib as O
 
     net = O.hhe6(params)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, visible, quota = host_cores()
 
     def run(n, nthreads):
         y = y0[: n * N_SPECIES].copy()
@@ -108,6 +152,7 @@ def cpu_baseline(params, y0, nH, seconds, tol, controller):
     n1 = int(max(8, min(NSYS, budget / max(per_sys, 1e-9))))
     dt1 = run(n1, 1)
     return {"value": n / dt, "unit": "systems/s", "cores": threads, "kind": "port",
+            "host_cpus_visible": visible, "cgroup_cpu_quota": quota,
             "sample": f"first {n} of the {NSYS} systems, one contiguous slice per thread; median of 3 runs, "
                       f"{dt:.1f} s each",
             "single_core": {"value": n1 / dt1, "unit": "systems/s", "cores": 1,
@@ -115,12 +160,88 @@ def cpu_baseline(params, y0, nH, seconds, tol, controller):
             "algorithm": ("reference controller" if controller == 0 else "adaptive controller") + f", tol {tol:g}"}
 
 
+# ------------------------------------------------------ NR_test (config 1)
+def nr_test_c1(W, reps=5):
+    """The reference's stiff linear test system: 1024 systems, N = 2, t in
+    [0, 1], dt0 = 1e-3, abs = rel = 1e-3, SIE.  GPU: the drop-in
+    cudaIntegrateSystem of wind_amd/lib/nr_test/sie.so (exact mode, host
+    arrays, as WIND's harness calls it); CPU: the oracle (c_baseline's
+    algorithm) on one core, as the reference's goldIntegrateSystem runs.
+    Accuracy against the closed form y1 = 2 e^-at - e^-1000at,
+    y2 = -e^-at + e^-1000at."""
+    import ctypes as C
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    n = 1024
+    y0, c, a = W.nr_test_systems(n, seed=0)
+    so = C.CDLL(os.path.join(ROOT, "wind_amd", "lib", "nr_test", "sie.so"))
+    fn = getattr(so, "_Z19cudaIntegrateSystemffiPfS_iiff")
+    fn.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float]
+    ts, y = [], None
+    for _ in range(reps + 1):
+        y = y0.copy()
+        t = time.perf_counter()
+        fn(0.0, 1.0, 1000, c.ctypes.data, y.ctypes.data, n, 2, 1e-3, 1e-3)
+        ts.append(time.perf_counter() - t)
+    gpu_s = float(np.median(ts[1:]))
+    yr = y0.copy()
+    t = time.perf_counter()
+    O.gold_integrate(O.SIE, O.nr_test(), 0.0, 1.0, 1000, c, yr, n, 1e-3, 1e-3, const_stride=4)
+    cpu_s = time.perf_counter() - t
+    exact = W.nr_test_exact(a, 1.0).reshape(-1, 2)
+    e = np.max(np.abs(y.reshape(-1, 2) - exact) / np.maximum(np.abs(exact), 1e-3), axis=1)
+    return {"workload": "NR_test stiff linear system, 1024 systems, N = 2, t in [0, 1], dt0 = 1e-3, "
+                        "abs = rel = 1e-3, SIE, reference controller",
+            "gpu": {"value": n / gpu_s, "unit": "systems/s", "ms_per_call": gpu_s * 1e3,
+                    "path": "drop-in cudaIntegrateSystem (nr_test/sie.so), exact mode, host arrays in and out"},
+            "cpu": {"value": n / cpu_s, "unit": "systems/s", "cores": 1, "kind": "port",
+                    "sample": "all 1024 systems, one thread (the reference's goldIntegrateSystem is serial)"},
+            "bit_identical_to_oracle": bool(np.array_equal(y.view(np.uint32), yr.view(np.uint32))),
+            "accuracy_vs_closed_form": {"max_rel_err": float(e.max()), "median_rel_err": float(np.median(e)),
+                                        "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3))}}
+
+
+# ------------------------------------------------------- hhe6 RK2 (config 2)
+def rk2_hhe6(torch, api, W, net, y0, cst, nsys, tol, reps=2):
+    """The RK2 leg of the hhe6 workload: the same systems, fast lane kernel,
+    adaptive controller at the headline tolerance."""
+    y = torch.empty_like(y0)
+    steps = torch.zeros(nsys, dtype=torch.int32, device=y0.device)
+    status = torch.zeros(nsys, dtype=torch.int32, device=y0.device)
+    ts, kern = [], []
+    tot = 0
+    for _ in range(reps):
+        y.copy_(y0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        tot, kms = api.integrate(net, 0.0, W.HHE6_TEND_YR, 1, cst, y, nsys, tol, tol, method=api.RK2,
+                                 mode=api.MODE_FAST, kernel=api.KERNEL_LANE, device_pointers=True, steps_out=steps,
+                                 status_out=status, want_kernel_ms=True, controller=api.CTRL_ADAPTIVE)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+        kern.append(kms)
+    st = status.cpu().numpy()
+    sub = steps.cpu().numpy().astype(np.int64)
+    acc = accuracy(y.cpu().numpy().reshape(-1, N_SPECIES), nsys)
+    return {"workload": f"hhe6, {nsys} systems, 1 Myr, RK2 (explicit), fast lane kernel, adaptive controller, "
+                        f"abs = rel = {tol:g}",
+            "value": nsys / min(ts), "unit": "systems/s", "kernel_ms": float(min(kern)), "substeps": int(tot),
+            "substeps_per_system": {"mean": float(sub.mean()), "max": int(sub.max())},
+            "status": {"ok": int(np.sum(st == 0)), "stalled": int(np.sum(st == 1)),
+                       "exhausted": int(np.sum(st == 2))},
+            "accuracy": acc}
+
+
 # ----------------------------------------------------------- mixed-N batch
 MIXED_NSYS = 1 << 18
 # Mass-action controller tolerances: every system of the FP64 sample ends
 # within 1e-3 (at rel 1e-4 the worst N = 5 system ends at 1.2e-3).
 MA_ABS, MA_REL = 1e-10, 8e-5
-PLAIN_SAMPLE = {5: 8192, 10: 8192, 32: 4096, 64: 2048, 157: 256}
+# Plain-kernel samples per bucket: at least 8 workgroups per CU (2048
+# systems on 256 CUs) so that the rate is the device's, not one system's.
+PLAIN_SAMPLE = {5: 8192, 10: 8192, 32: 4096, 64: 2048, 157: 2048}
 
 
 def dense_flops(N):
@@ -242,6 +363,7 @@ def dense_roofline(torch, api, W, nsys):
                           device_pointers=True, controller=api.CTRL_ADAPTIVE, want_kernel_ms=True,
                           max_attempts=100000, cost_hint=steps)
     return {"workload": f"N = 157 mass-action, first {nsys} of {W.DENSE_NSYS} systems, t in [0, 1e9]",
+            "sample": f"first {nsys} of the {W.DENSE_NSYS} systems of the N = 157 workload",
             "kernel": "ma_dense_integrate<SIE, MFMA_F32>", "systems_per_s": nsys / (kms / 1e3),
             "kernel_ms": kms, "substeps": int(tot), "flops_per_substep": F,
             "us_per_substep_per_cu": kms * 1e3 / (float(tot) / 256.0),
@@ -288,9 +410,11 @@ def dense_precision(torch, api, W, nsys):
 # ------------------------------------------------------------------ serving
 def serving(api, W, seconds, load_frac, seed=5):
     """Continuous-batching request server: resident cell blocks of mixed
-    networks; Poisson arrivals at `load_frac` of the capacity measured with a
-    closed-loop burst; each request advances one block by dt and returns its
-    state to the host."""
+    networks; Poisson arrivals at 50%, `load_frac` and 90% of the capacity
+    measured with a closed-loop burst, `seconds` each; each request advances
+    one block by dt and returns its state to the host.  The headline serving
+    figures are those at `load_frac`; each load also reports what its
+    slowest 1% of requests carry."""
     rng = np.random.default_rng(seed)
     hp = W.hhe6_params()
     nets = {"hhe6": api.Network.builtin(api.NET_HHE6, hp)}
@@ -332,35 +456,65 @@ def serving(api, W, seconds, load_frac, seed=5):
     burst_s = time.perf_counter() - t
     sys_per_req = float(np.mean([b["n"] for b in blocks]))
     cap_req_s = len(ids) / burst_s
-    rate = load_frac * cap_req_s
-    # open loop: Poisson arrivals for `seconds`
-    lat, served, pending = [], 0, []
-    t_start = time.perf_counter()
-    t_next = t_start
-    while True:
-        now = time.perf_counter()
-        if now - t_start >= seconds:
-            break
-        if now >= t_next:
-            b = blocks[int(rng.integers(len(blocks)))]
-            pending.append((submit(b), b["n"]))
-            t_next += rng.exponential(1.0 / rate)
-        else:
-            time.sleep(min(0.0005, t_next - now))
-    for rid, n in pending:
-        l_us, _ = srv.wait(rid)
-        lat.append(l_us / 1e3)
-        served += n
-    wall = time.perf_counter() - t_start
+    def open_loop(frac):
+        """Poisson arrivals at `frac` of the measured capacity for `seconds`;
+        returns per-request (latency ms, systems, substeps)."""
+        rate = frac * cap_req_s
+        reqs = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        while True:
+            now = time.perf_counter()
+            if now - t_start >= seconds:
+                break
+            if now >= t_next:
+                b = blocks[int(rng.integers(len(blocks)))]
+                reqs.append((submit(b), b))
+                t_next += rng.exponential(1.0 / rate)
+            else:
+                time.sleep(min(0.0005, t_next - now))
+        out = []
+        for rid, b in reqs:
+            l_us, steps = srv.wait(rid)
+            out.append((l_us / 1e3, b["n"], steps, b["kind"]))
+        return out, time.perf_counter() - t_start
+
+    loads = {}
+    for frac in sorted({0.5, load_frac, 0.9}):
+        res, wall = open_loop(frac)
+        lat = np.array([r[0] for r in res])
+        nsys = np.array([r[1] for r in res], np.float64)
+        work = np.array([r[2] for r in res], np.float64)
+        p99 = float(np.quantile(lat, 0.99))
+        tail = lat >= p99
+        loads[f"{frac:g}"] = {
+            "requests": int(len(lat)), "systems_per_s": float(nsys.sum() / wall),
+            "latency_ms": {"p50": float(np.quantile(lat, 0.5)), "p90": float(np.quantile(lat, 0.9)), "p99": p99,
+                           "max": float(lat.max())},
+            # the slowest 1%: how much work they carry against the average request
+            "p99_tail": {"requests": int(tail.sum()), "mean_systems": float(nsys[tail].mean()),
+                         "mean_substeps": float(work[tail].mean()),
+                         "substeps_vs_mean_request": float(work[tail].mean() / max(work.mean(), 1.0)),
+                         "kinds": sorted({r[3] for r, t in zip(res, tail) if t})},
+            "latency_vs_work_spearman": float(_spearman(lat, work)),
+        }
     srv.close()
-    lat = np.array(lat)
+    main_load = loads[f"{load_frac:g}"]
     return {"workload": "24 resident cell blocks (2/3 hhe6 with logU[1k, 256k] systems and dt logU[1e3, 1e6] yr; "
                         "1/3 mixed-N mass-action with logU[1k, 256k]/max(1, (N/10)^2) systems and dt logU[1e6, 1e9]); "
                         "each request advances one block and copies its state to the host",
-            "slots": 8, "capacity_requests_per_s": cap_req_s, "offered_load": load_frac,
-            "requests": int(len(lat)), "systems_per_s": served / wall, "mean_systems_per_request": sys_per_req,
-            "latency_ms": {"p50": float(np.quantile(lat, 0.5)), "p99": float(np.quantile(lat, 0.99)),
-                           "max": float(lat.max())}}
+            "slots": 8, "capacity_requests_per_s": cap_req_s, "mean_systems_per_request": sys_per_req,
+            "offered_load": load_frac, "requests": main_load["requests"],
+            "systems_per_s": main_load["systems_per_s"], "latency_ms": main_load["latency_ms"],
+            "seconds_per_load": seconds, "by_offered_load": loads}
+
+
+def _spearman(a, b):
+    ra = np.argsort(np.argsort(a)).astype(np.float64)
+    rb = np.argsort(np.argsort(b)).astype(np.float64)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    return float((ra * rb).sum() / np.sqrt((ra * ra).sum() * (rb * rb).sum()))
 
 
 def main():
@@ -378,9 +532,11 @@ def main():
     ap.add_argument("--mixed-nsys", type=int, default=MIXED_NSYS)
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-rk2", action="store_true", help="skip the RK2 leg of the hhe6 workload")
+    ap.add_argument("--no-c1", action="store_true", help="skip the NR_test (1024 systems) GPU and CPU row")
     ap.add_argument("--no-generated", action="store_true",
                     help="mixed batch: table-driven networks for every N (no generated plugins)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="open-loop seconds per offered load")
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--dense-nsys", type=int, default=4096)
@@ -456,16 +612,30 @@ def main():
         acc["error_definition"] = "max over species |y - y_fp64| / max(|y_fp64|, 1e-3)"
 
     algo_bytes = BYTES_PER_SYSTEM * nsys
-    algo_flops = F_SUBSTEP * float(tot)
+    kname = "lane_fast_kernel<NetHHe6, SIE>" if mode == api.MODE_FAST else "lane_integrate<NetHHe6, SIE, MathExact>"
+    # FP32 work per sub-step: executed (from the FP32 instruction counters
+    # of the same launch, profiles/valu_counts.json) when measured for this
+    # kernel and tolerance, else the algorithmic count.
+    f_sub, f_src = F_SUBSTEP, "algorithmic (2 network evaluations and 3 solves per attempt)"
+    vcp = os.path.join(ROOT, "profiles", "valu_counts.json")
+    if os.path.exists(vcp):
+        with open(vcp) as f:
+            vc = json.load(f).get(f"lane_hhe6_{a.mode}")
+        if vc and abs(vc.get("tol", tol) - tol) < 1e-12:
+            f_sub, f_src = vc["fp32_flops_per_substep"], "executed FP32 operations per sub-step (PMC, " + vcp[len(ROOT) + 1:] + ")"
+    flops = f_sub * float(tot)
     roofline = {
-        "bound": "hbm", "achieved": algo_bytes / kernel_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": algo_bytes / kernel_s / 1e9 / PEAK_HBM_GBS, "traffic": None,
-        "kernel": ("lane_fast_kernel<NetHHe6, SIE>" if mode == api.MODE_FAST else "lane_integrate<NetHHe6, SIE, MathExact>"), "kernel_ms": kernel_s * 1e3,
-        "valu": {"achieved": algo_flops / kernel_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                 "frac": algo_flops / kernel_s / 1e12 / PEAK_FP32_TFLOPS, "flops_per_substep": F_SUBSTEP},
-        # the roofline that binds this kernel: FP32 vector ALU (no MFMA work;
-        # the HBM figures above are the state in and out, small by design)
-        "binding": "valu",
+        # the roofline that binds this kernel: the FP32 vector ALU (it issues
+        # no MFMA work and its HBM traffic is the state in and out)
+        "bound": "valu", "achieved": flops / kernel_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": flops / kernel_s / 1e12 / PEAK_FP32_TFLOPS, "traffic": None,
+        "kernel": kname, "kernel_ms": kernel_s * 1e3,
+        "kernel_ms_includes": "the start-order pre-pass (lane_activity, hint_* kernels, ~0.15 ms)",
+        "flops_per_substep": f_sub, "flops_source": f_src,
+        "frac_of_unpacked_fp32_peak": flops / kernel_s / 1e12 / PEAK_FP32_UNPACKED_TFLOPS,
+        "hbm": {"achieved": algo_bytes / kernel_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": algo_bytes / kernel_s / 1e9 / PEAK_HBM_GBS,
+                "algorithmic_bytes_per_system": BYTES_PER_SYSTEM},
     }
     trp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(trp):
@@ -559,6 +729,8 @@ def main():
                            "launch shape), same controller and tolerance as the headline",
                  "speedup": value / pv}
 
+    rk2 = None if a.no_rk2 else rk2_hhe6(torch, api, W, net, y0, cst, nsys, tol)
+    c1 = None if a.no_c1 else nr_test_c1(W)
     mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain,
                                                 not a.no_generated)
     dense = None if a.no_dense else dense_roofline(torch, api, W, a.dense_nsys)
@@ -571,7 +743,10 @@ def main():
         "metric": "ODE systems/sec @ rel err<=1e-3 vs FP64; p50/p99 request latency",
         "value": value, "unit": "systems/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (error test f64)", "data": "synthetic (seeded)",
+        "vs_baseline": None, "dtype": "f32" if mode == api.MODE_FAST else "f32 (error test f64)",
+        "dtype_note": ("fast lane kernel: state, arithmetic and the adaptive error norm in float"
+                       if mode == api.MODE_FAST else "exact mode: float state, relative error test in double"),
+        "data": "synthetic (seeded)",
         "config": {"workload": "hhe6 H/He ionisation + energy, 6 ODEs, 2^20 systems, 1 Myr, SIE, "
                                f"abs=rel={tol:g}", "nsys": nsys, "method": "SIE", "mode": a.mode,
                    "controller": a.controller, "parallelism": "one GPU"},
@@ -580,7 +755,7 @@ def main():
         "substeps_per_system": {"mean": float(sub.mean()), "max": int(sub.max())},
         "status": {"ok": int(np.sum(st == 0)), "stalled": int(np.sum(st == 1)), "exhausted": int(np.sum(st == 2))},
         "accuracy": acc, "with_cost_hint": hinted, "looser_tolerances": sweep, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
-        "reference_algorithm": ref_alg, "plain_gpu": plain,
+        "reference_algorithm": ref_alg, "plain_gpu": plain, "rk2": rk2, "nr_test_1024": c1,
         "cpu_baseline": cpu, "cpu_same_algorithm": cpu_same,
         "mixed_batch": mixed, "dense_157": dense, "dense_157_precision": precision, "serving": serve,
     }

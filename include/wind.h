@@ -145,6 +145,39 @@ int wind_network_create_mass_action(int N, int R, const int32_t *rx, const float
 void wind_network_destroy(wind_network *net);
 int wind_network_size(const wind_network *net); /* N */
 
+/* Per-attempt trace of wind_integrate (extension).  The reference prints
+ * the same quantities for one system in its DEBUGBLOCK builds (the full-step
+ * and two-half-step states of every attempt: cuda/common/kernel.cu:214-249,
+ * cuda/c_baseline/common/common_gold.c:102-113).  Attempt k of system s is
+ * at s * capacity + k; states at (s * capacity + k) * N.
+ *   Record (optional): y1 / y2 receive each attempt's full-step and
+ *   two-half-step states, flag the kernel's own error test (1 = its
+ *   controller rejects the attempt), for the first `capacity` attempts.
+ *   Replay (h, accept and count together): attempt k takes the step h[...]
+ *   (the reference's timestep after its fmin clamp, common_gold.c:77) and
+ *   is accepted iff accept[...] != 0, in place of the controller's choice;
+ *   system s stops after count[s] <= capacity attempts.  An accepted attempt
+ *   advances tnow by h and keeps what the controller would keep (reference
+ *   SIE: y2; otherwise 2 y2 - y1).  With the oracle's own sequence the
+ *   kernel integrates the same accepted-step sequence as the oracle.
+ *   y_start (optional, with replay): attempt k starts from the state
+ *   y_start[(s * capacity + k) * N ...] instead of the kernel's own, so that
+ *   each attempt can be compared with a reference's from the same state.
+ * Pointers are host or device like the other arrays (device_pointers).
+ * Supported by the lane kernels of the compiled-in networks (exact and fast
+ * mode) and by the dense solver of mass-action networks (fast mode);
+ * other kernels return WIND_ERR_UNSUPPORTED. */
+typedef struct wind_trace {
+    int capacity;            /* attempts per system in each array (> 0) */
+    const float *h;          /* replay: [Nsystems * capacity] */
+    const int32_t *accept;   /* replay: [Nsystems * capacity] */
+    const int32_t *count;    /* replay: [Nsystems] */
+    const float *y_start;    /* replay, optional: [Nsystems * capacity * N] */
+    float *y1;               /* record: [Nsystems * capacity * N] */
+    float *y2;               /* record: [Nsystems * capacity * N] */
+    int32_t *flag;           /* record: [Nsystems * capacity] */
+} wind_trace;
+
 typedef struct wind_options {
     int method;            /* wind_method */
     int mode;              /* wind_mode */
@@ -172,12 +205,20 @@ typedef struct wind_options {
                               decreasing order of it, so that long ones do not
                               finish alone at the end of the run.  Results do
                               not depend on it. */
+    const wind_trace *trace;  /* wind_integrate: optional per-attempt trace or
+                              replay (see wind_trace); NULL = none */
 } wind_options;
 
 /* wind_options.flags */
 enum wind_flags {
     WIND_FLAG_GRAPH = 1,       /* batch: capture the fork/join launch into a hipGraph and replay it */
-    WIND_FLAG_NO_AUTOTUNE = 2  /* batch: use default kernel shapes instead of tuning each bucket */
+    WIND_FLAG_NO_AUTOTUNE = 2, /* batch: use default kernel shapes instead of tuning each bucket */
+    WIND_FLAG_PIVOT = 4        /* fast mode: partial pivoting in the LU solves (the row with the
+                                  largest magnitude in the pivot column; a wave-shuffle search in
+                                  the one-workgroup-per-system kernels, selects in the
+                                  lane-per-system ones).  Mass-action networks then use the LDS
+                                  LU instead of the blocked MFMA solver.  Exact mode rejects it:
+                                  the reference's Gauss-Jordan is unpivoted. */
 };
 
 void wind_options_init(wind_options *opts);

@@ -4,6 +4,12 @@
                                   every 2048th, integrated with scipy's Radau IIA
                                   (order 5) at rtol 1e-10, atol 1e-14 on the float64
                                   network statement in tests/hhe6_f64.py.
+    tests/golden/truth_hhe6_tail.npz
+                                  the same workload's accuracy tail: 8192 systems, 6144 drawn
+                                  stratified over (log T, log n_H) (24 from each cell of a 16 x 16
+                                  grid) and the 2048 with the most sub-steps in the oracle's
+                                  float32 run at the benchmark's tolerance (adaptive controller,
+                                  abs = rel = 2e-6); same Radau settings.
     tests/golden/truth_mass_action.npz
                                   mixed-N workload (2^18 systems, seed 4, t in [0, 1e9]):
                                   32 evenly spaced systems of each N bucket; and the
@@ -11,7 +17,7 @@
                                   every 2048th.  Radau IIA at rtol 1e-10, atol 1e-20
                                   with the analytic Jacobian of tests/ma_f64.py.
 
-    python tests/golden/make_truth.py [--procs 8] [--only hhe6|mass_action]
+    python tests/golden/make_truth.py [--procs 8] [--only hhe6|hhe6_tail|mass_action]
 
 The benchmark compares its final states for the same systems against these
 arrays (final-state relative error vs FP64).
@@ -67,6 +73,56 @@ def hhe6_truth(procs):
     print("wrote", out, "failed solves:", int((status != 0).sum()))
 
 
+TAIL_GRID, TAIL_PER_CELL, TAIL_LONGEST, TAIL_TOL = 16, 24, 2048, 2e-6
+
+
+def hhe6_tail_selection(y0, nH, steps, seed=11):
+    """Indices of the tail sample: TAIL_PER_CELL systems from each cell of a
+    TAIL_GRID x TAIL_GRID grid over (log10 T, log10 n_H) (all of a cell's
+    systems if it has fewer), then the TAIL_LONGEST systems with the most
+    sub-steps not already chosen.  T is recovered from the internal energy,
+    e = 1.5 mu T / 1e4."""
+    rng = np.random.default_rng(seed)
+    Y = y0.reshape(-1, 6).astype(np.float64)
+    xe = Y[:, 1] + Y[:, 3] + 2.0 * Y[:, 4]
+    mu = Y[:, 0:5].sum(axis=1) + xe
+    T = Y[:, 5] * 1.0e4 / (1.5 * mu)
+    bt = np.clip(((np.log10(T) - 2.0) / 6.0 * TAIL_GRID).astype(int), 0, TAIL_GRID - 1)
+    bn = np.clip(((np.log10(nH) + 4.0) / 8.0 * TAIL_GRID).astype(int), 0, TAIL_GRID - 1)
+    cell = bt * TAIL_GRID + bn
+    chosen = []
+    for cidx in range(TAIL_GRID * TAIL_GRID):
+        members = np.nonzero(cell == cidx)[0]
+        if len(members):
+            chosen.append(rng.choice(members, size=min(TAIL_PER_CELL, len(members)), replace=False))
+    strat = np.unique(np.concatenate(chosen))
+    order = np.argsort(-steps, kind="stable")
+    longest = [i for i in order[: TAIL_LONGEST + len(strat)] if i not in set(strat.tolist())][:TAIL_LONGEST]
+    return np.sort(strat), np.array(longest, np.int64)
+
+
+def hhe6_tail_truth(procs):
+    import oracle_lib as O
+
+    p = W.hhe6_params()
+    y0, nH = W.hhe6_systems(HHE6_NSYS, seed=HHE6_SEED)
+    y = y0.copy()
+    _, steps, _ = O.gold_integrate(O.SIE, O.hhe6(p), 0.0, W.HHE6_TEND_YR, 1, nH, y, HHE6_NSYS, TAIL_TOL, TAIL_TOL,
+                                   const_stride=1, threads=procs, controller=1)
+    strat, longest = hhe6_tail_selection(y0, nH, steps)
+    idx = np.concatenate([strat, longest])
+    Y0 = y0.reshape(-1, 6).astype(np.float64)
+    jobs = [(Y0[i], float(nH[i]), p.astype(np.float64), W.HHE6_TEND_YR) for i in idx]
+    with Pool(procs) as pool:
+        res = pool.map(_solve, jobs, chunksize=8)
+    status = np.array([r[0] for r in res], np.int32)
+    yT = np.array([r[1] for r in res])
+    out = os.path.join(HERE, "truth_hhe6_tail.npz")
+    np.savez_compressed(out, idx=idx, n_stratified=len(strat), y=yT, status=status, tend=W.HHE6_TEND_YR,
+                        seed=HHE6_SEED, nsys=HHE6_NSYS, params=p, oracle_substeps=steps[idx])
+    print("wrote", out, "systems:", len(idx), "failed solves:", int((status != 0).sum()))
+
+
 MIXED_NSYS = 1 << 18
 MIXED_SEED = 4
 PER_BUCKET = 32
@@ -117,9 +173,11 @@ def mass_action_truth(procs):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--procs", type=int, default=8)
-    ap.add_argument("--only", choices=["hhe6", "mass_action"])
+    ap.add_argument("--only", choices=["hhe6", "hhe6_tail", "mass_action"])
     a = ap.parse_args()
-    if a.only != "mass_action":
+    if a.only in (None, "hhe6"):
         hhe6_truth(a.procs)
-    if a.only != "hhe6":
+    if a.only in (None, "hhe6_tail"):
+        hhe6_tail_truth(a.procs)
+    if a.only in (None, "mass_action"):
         mass_action_truth(a.procs)

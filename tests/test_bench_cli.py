@@ -22,8 +22,10 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_roofline_constants():
-    # hhe6: network 674 flops + 2N^2 + 2/3 N^3 + 2N^2 + 2N at N = 6
-    assert bench.F_SUBSTEP == 674 + 72 + 144 + 72 + 12 == 974
+    # hhe6 per solve: 2N^2 + 2/3 N^3 + 2N^2 + 2N at N = 6; per attempt two
+    # network evaluations (674 flops each) and three solves over 3 sub-steps
+    assert bench.F_SOLVE == 72 + 144 + 72 + 12 == 300
+    assert bench.F_SUBSTEP == (2 * 674 + 3 * 300) / 3
     # state in and out (2N floats) and one constant per system
     assert bench.BYTES_PER_SYSTEM == 4 * (2 * 6 + 1) == 52
     # mass action at N = 157: 64N + 4N^2 + 2/3 N^3 + 2N
@@ -36,6 +38,7 @@ def test_cpu_baseline_report():
     out = bench.cpu_baseline(p, y0, nH, 0.4, 1e-3, 0)
     assert out["kind"] == "port" and out["unit"] == "systems/s"
     assert out["cores"] >= 1 and out["value"] > 0
+    assert out["cores"] <= out["host_cpus_visible"] == len(os.sched_getaffinity(0))
     assert out["single_core"]["cores"] == 1 and out["single_core"]["value"] > 0
     assert "median of 3" in out["sample"]
     assert np.isfinite(out["value"])

@@ -121,7 +121,7 @@ def test_plugin_descriptor_and_counts(name):
     lib = C.CDLL(os.path.join(GEN, name, "libwind_net.so"))
     lib.wind_plugin_get.restype = C.POINTER(Desc)
     d = lib.wind_plugin_get().contents
-    assert d.abi == 2 and d.N == g["N"] and d.nconst == g["nconst"] and d.nreactions == g["R"]
+    assert d.abi == 3 and d.N == g["N"] and d.nconst == g["nconst"] and d.nreactions == g["R"]
     assert d.name.decode() == name and d.flops_rhs == g["flops_rhs"] and d.flops_jac == g["flops_jac"]
     assert d.launch and d.lockstep_attempt
 
@@ -191,3 +191,93 @@ def test_tile_expansion():
     assert t["species"] == 4 and [r["reactants"] for r in t["reactions"]] == [[0, 1], [2, 3]]
     assert [r["products"] for r in t["reactions"]] == [[1], [3]]
     assert codegen.generate(spec)["N"] == 4
+
+
+# ------------------------------------------------------- expression networks
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "c_baseline_kats.npz"))
+
+
+def test_generated_katz96_equals_reference_golden():
+    """Katz96 generated from its expressions (networks.katz96_spec) computes
+    the reference's compiled calculate_dydt / calculate_jacobian bits
+    (golden vectors from cuda/ode_system/ode_gold.c built from source) and
+    the oracle's Katz96, in float."""
+    gen = O.generated("katz96_gen")
+    ref = O.katz96()
+    for y, c, f, J in zip(G["katz_y"], G["katz_c"], G["katz_f"], G["katz_J"]):
+        fg, Jg = _eval(gen, y, c, np.float32)
+        assert np.array_equal(fg.view(np.uint32), np.asarray(f, np.float32).view(np.uint32))
+        assert np.array_equal(Jg.view(np.uint32), np.asarray(J, np.float32).view(np.uint32))
+        fo, Jo = _eval(ref, y, c, np.float32)
+        assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
+        assert np.array_equal(Jg.view(np.uint32), Jo.view(np.uint32))
+
+
+def test_generated_nr_test_equals_oracle():
+    gen = O.generated("nr_test_gen")
+    ref = O.nr_test()
+    y0, c, _ = W.nr_test_systems(16, seed=2)
+    rng = np.random.default_rng(3)
+    for s in range(16):
+        y = rng.uniform(-2.0, 2.0, 2).astype(np.float32)
+        cs = c[4 * s:4 * s + 4]
+        for dtype in (np.float32, np.float64):
+            fg, Jg = _eval(gen, y, cs, dtype)
+            fo, Jo = _eval(ref, y, cs, dtype)
+            assert np.array_equal(fg.view(np.uint8), fo.view(np.uint8))
+            assert np.array_equal(Jg.view(np.uint8), Jo.view(np.uint8))
+
+
+@pytest.mark.parametrize("name,T", [("katz96_t4", 4), ("katz96_t50", W.KATZ96_NTILE)])
+def test_tiled_katz96(name, T):
+    """Katz96 tiled T times as the reference tiles it (ode_base.py:86-88,
+    215-242): with every tile holding the same state (the reference tiles
+    its initial conditions, np.tile), each tile's RHS and Jacobian block are
+    the untiled network's bit for bit, the Jacobian is block diagonal, and
+    the electron density is the first tile's (the reference's
+    dconstants_string is inserted once, ahead of the per-tile code)."""
+    one, tiled = O.generated("katz96_gen"), O.generated(name)
+    assert tiled.N == 5 * T
+    for y, c in zip(G["katz_y"][:6], G["katz_c"][:6]):
+        yt = np.tile(np.asarray(y, np.float32), T)
+        f1, J1 = _eval(one, y, c, np.float32)
+        fT, JT = _eval(tiled, yt, c, np.float32)
+        JT = JT.reshape(5 * T, 5 * T)  # JT[col, row]
+        for t in range(T):
+            assert np.array_equal(fT[5 * t:5 * t + 5].view(np.uint32), f1.view(np.uint32))
+            blk = JT[5 * t:5 * t + 5, 5 * t:5 * t + 5].reshape(-1)
+            assert np.array_equal(blk.view(np.uint32), J1.view(np.uint32))
+        assert not JT[np.kron(np.eye(T), np.ones((5, 5))) == 0].any()
+    # a different state in the last tile: its rates still use the first tile's ne
+    y = np.asarray(G["katz_y"][0], np.float32)
+    c = G["katz_c"][0]
+    yt = np.tile(y, T)
+    yt[-5:] = np.asarray(G["katz_y"][1], np.float32)
+    fT, _ = _eval(tiled, yt, c, np.float32)
+    ne0 = np.float32(np.float64(np.float32(y[1] + y[3])) + np.float64(y[4]) * 2.0)
+    y1 = yt[-5:]
+    want = np.float32(np.float32(np.float32(c[2] * ne0) * y1[1]) -
+                      np.float32(np.float32(np.float32(c[0] * ne0) + c[1]) * y1[0]))
+    assert fT[-5] == want
+
+
+def test_expression_network_counts_and_checks():
+    g = codegen.generate(W.katz96_spec())
+    assert g["N"] == 5 and g["nconst"] == 10
+    # ne: 3 operations; RHS 6 + 6 + 7 + 14 + 6
+    assert g["flops_rhs"] == 42
+    g50 = codegen.generate(W.katz96_spec("k", 50))
+    assert g50["N"] == 250 and g50["flops_rhs"] == 39 * 50 + 3
+    bad = dict(W.katz96_spec("bad"))
+    bad["rhs"] = list(bad["rhs"][:-1]) + ["y[5] * c[0]"]
+    with pytest.raises(ValueError, match="out of range"):
+        codegen.generate(bad)
+    bad["rhs"] = list(bad["rhs"][:-1]) + ["nf * y[0]"]
+    with pytest.raises(ValueError, match="unknown name"):
+        codegen.generate(bad)
+    bad["rhs"] = list(bad["rhs"][:-1]) + ["J[0] * y[0]"]
+    with pytest.raises(ValueError, match="only readable in Jacobian"):
+        codegen.generate(bad)
+    bad["rhs"] = list(bad["rhs"][:-1]) + ["(y[0] * c[0]"]
+    with pytest.raises(ValueError, match="parentheses"):
+        codegen.generate(bad)

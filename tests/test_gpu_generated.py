@@ -176,3 +176,114 @@ def test_tiled_network_reproduces_untiled_per_tile():
     O.gold_integrate(api.SIE, O.generated("demo_decay_t3"), 0.0, 50.0, 10, c, yr, ns, 1e-4, 1e-4, const_stride=2,
                      max_attempts=20000)
     assert same(yb, yr)
+
+
+# ------------------------------------------------- networks from expressions
+G = np.load(os.path.join(ROOT, "tests", "golden", "c_baseline_kats.npz"))
+
+
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+@pytest.mark.parametrize("controller", [api.CTRL_REFERENCE, api.CTRL_ADAPTIVE])
+@pytest.mark.parametrize("kernel", [api.KERNEL_LANE, api.KERNEL_BLOCK])
+def test_generated_katz96_equals_builtin(method, controller, kernel):
+    """Katz96 generated from its expressions integrates to the compiled-in
+    network's bits (exact mode), which the oracle and the reference's
+    c_baseline pin."""
+    ys, cs = G["sys_y"], G["sys_c"]
+    ns = len(ys)
+    y0 = ys.reshape(-1).astype(np.float32).copy()
+    c = cs.reshape(-1).astype(np.float32).copy()
+    out = []
+    for net in (api.Network.plugin("katz96_gen"), api.Network.builtin(api.NET_KATZ96)):
+        y = y0.copy()
+        steps = np.zeros(ns, np.int32)
+        api.integrate(net, 0.0, 50.0, 1, c, y, ns, 1e-4, 1e-4, method=method, mode=api.MODE_EXACT, kernel=kernel,
+                      steps_out=steps, controller=controller, max_attempts=100000)
+        out.append((y, steps))
+    assert same(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_generated_nr_test_equals_builtin(method):
+    y0, c, _ = W.nr_test_systems(1024, seed=9)
+    out = []
+    for net in (api.Network.plugin("nr_test_gen"), api.Network.builtin(api.NET_NR_TEST)):
+        y = y0.copy()
+        steps = np.zeros(1024, np.int32)
+        api.integrate(net, 0.0, 1.0, 1000, c, y, 1024, 1e-3, 1e-3, method=method, steps_out=steps)
+        out.append((y, steps))
+    assert same(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("name,T,sel", [("katz96_t4", 4, slice(0, 12)), (f"katz96_t{W.KATZ96_NTILE}",
+                                                                        W.KATZ96_NTILE, slice(0, 2))])
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_tiled_katz96_exact_matches_oracle(name, T, sel, method):
+    """Katz96 tiled as in the reference's Ntile scaling study (N = 5 T,
+    T = 4 and 50): the one-workgroup-per-system kernel (matrices in LDS at
+    N = 20, in the device scratch at N = 250) reproduces the oracle's
+    c_baseline integration of the generated network bit for bit, and every
+    tile ends equal to the untiled network's result."""
+    ys, cs = G["sys_y"][sel], G["sys_c"][sel]
+    ns = len(ys)
+    y0 = np.concatenate([np.tile(y.astype(np.float32), T) for y in ys])
+    c = cs.reshape(-1).astype(np.float32).copy()
+    onet = O.generated(name)
+    yr = y0.copy()
+    tr, sr, str_ = O.gold_integrate(method, onet, 0.0, 50.0, 1, c, yr, ns, 5e-3, 5e-3, const_stride=10,
+                                    threads=ns) if method == api.SIE else \
+        O.gold_integrate(method, onet, 0.0, 50.0, 1, c, yr, ns, 5e-3, 5e-3, const_stride=10)
+    net = api.Network.plugin(name)
+    assert net.N == 5 * T
+    y = y0.copy()
+    steps = np.zeros(ns, np.int32)
+    tot, _ = api.integrate(net, 0.0, 50.0, 1, c, y, ns, 5e-3, 5e-3, method=method, mode=api.MODE_EXACT,
+                           steps_out=steps)
+    print(f"{name} m{method}: substeps gpu {tot} oracle {tr}", flush=True)
+    assert np.array_equal(steps, sr)
+    assert same(y, yr)
+    # every tile equals the untiled Katz96 (tiles are independent copies)
+    y1 = ys.reshape(-1).astype(np.float32).copy()
+    s1 = np.zeros(ns, np.int32)
+    api.integrate(api.Network.builtin(api.NET_KATZ96), 0.0, 50.0, 1, c, y1, ns, 5e-3, 5e-3, method=method,
+                  mode=api.MODE_EXACT, steps_out=s1)
+    # values (a zero's sign may differ: the zero blocks add signed zeros)
+    assert np.array_equal(y.reshape(ns, T, 5), np.broadcast_to(y1.reshape(ns, 1, 5), (ns, T, 5)))
+
+
+def test_tiled_katz96_fast_mode():
+    """Fast mode at N = 250 (LU in the device scratch): the tiles stay
+    identical to each other and end within 1e-3 of the exact-mode result."""
+    T = W.KATZ96_NTILE
+    ys, cs = G["sys_y"][:16], G["sys_c"][:16]
+    ns = len(ys)
+    y0 = np.concatenate([np.tile(y.astype(np.float32), T) for y in ys])
+    c = cs.reshape(-1).astype(np.float32).copy()
+    net = api.Network.plugin(f"katz96_t{T}")
+    yf, ye = y0.copy(), y0.copy()
+    api.integrate(net, 0.0, 50.0, 1, c, yf, ns, 1e-4, 1e-4, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE)
+    api.integrate(net, 0.0, 50.0, 1, c, ye, ns, 1e-4, 1e-4, mode=api.MODE_EXACT, controller=api.CTRL_ADAPTIVE)
+    F = yf.reshape(ns, T, 5)
+    assert np.array_equal(F, np.broadcast_to(F[:, :1], F.shape))
+    E = ye.reshape(ns, T, 5).astype(np.float64)
+    err = np.abs(F - E) / np.maximum(np.abs(E), 1e-3)
+    assert err.max() <= 1e-3, err.max()
+
+
+def test_tiled_katz96_drop_in_symbol():
+    """The tiled network's drop-in sie.so (its NUM_CONST = 10) gives the
+    oracle's bits through the reference's entry point."""
+    T = 4
+    ys, cs = G["sys_y"][:8], G["sys_c"][:8]
+    ns = len(ys)
+    y0 = np.concatenate([np.tile(y.astype(np.float32), T) for y in ys])
+    c = cs.reshape(-1).astype(np.float32).copy()
+    so = C.CDLL(os.path.join(GEN, "katz96_t4", "sie.so"))
+    fn = getattr(so, "_Z19cudaIntegrateSystemffiPfS_iiff")
+    fn.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float]
+    y = y0.copy()
+    k = fn(0.0, 50.0, 1, c.ctypes.data, y.ctypes.data, ns, 5 * T, 5e-3, 5e-3)
+    yr = y0.copy()
+    kr, _, _ = O.gold_integrate(O.SIE, O.generated("katz96_t4"), 0.0, 50.0, 1, c, yr, ns, 5e-3, 5e-3,
+                                const_stride=10)
+    assert k == kr and same(y, yr)

@@ -126,6 +126,44 @@ def test_mass_action_exact(N, method):
     check(g, r, f"mass_action N{N} m{method}")
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_hhe6_rk2_exact_1myr(kernel):
+    """The RK2 leg of the hhe6 workload as the reference runs it: explicit
+    RK2 with the reference controller (tol 1e-3) over the full 1 Myr, on 256
+    of the FP64-fixture systems, with a 20000-attempt cap.  Many systems
+    force-accept unstable steps and overflow, or reach the cap: status,
+    per-system attempt counts and bits (NaN positions included) must equal
+    the oracle's."""
+    p = W.hhe6_params()
+    d = np.load(os.path.join(ROOT, "tests", "golden", "truth_hhe6.npz"))
+    y0_all, nH_all = W.hhe6_systems(int(d["nsys"]), seed=int(d["seed"]))
+    idx = d["idx"][:256]
+    y0 = np.ascontiguousarray(y0_all.reshape(-1, 6)[idx].reshape(-1))
+    nH = np.ascontiguousarray(nH_all[idx])
+    net = api.Network.builtin(api.NET_HHE6, p)
+    g = run_gpu(net, 0.0, 1.0e6, 1, nH, y0, 256, 1e-3, api.RK2, kernel, max_attempts=20000)
+    r = run_oracle(O.hhe6(p), 0.0, 1.0e6, 1, nH, y0, 256, 1e-3, api.RK2, 1, max_attempts=20000)
+    check(g, r, f"hhe6 RK2 1 Myr k{kernel}")
+    assert (r[2] == api.STATUS_EXHAUSTED).any() and (r[2] == api.STATUS_OK).any()
+
+
+@pytest.mark.parametrize("N,ns,tend", [(64, 8, 1.0e6), (157, 4, 1.0e5)])
+@pytest.mark.parametrize("method", [api.SIE, api.RK2])
+def test_mass_action_exact_large(N, ns, tend, method):
+    """Exact mode at the mixed workload's larger sizes, the one-workgroup-per-
+    system Gauss-Jordan kernel of the batch's plain denominator (N = 157: its
+    two matrices exceed LDS and live in the device scratch), bit for bit
+    against the oracle."""
+    rx, k = (W.mixed_network(N) if N != W.DENSE_N else W.mass_action_network(N, seed=W.DENSE_NET_SEED))
+    y0, sc = W.mass_action_systems(ns, N, seed=77 + N, scale_range=(0.5, 2.0))
+    net = api.Network.mass_action(N, rx, k)
+    g = run_gpu(net, 0.0, tend, 1, sc, y0, ns, 1e-3, method, api.KERNEL_AUTO)
+    y = y0.copy()
+    tot, steps, _ = O.gold_integrate(method, O.mass_action(N, rx, k), 0.0, tend, 1, sc, y, ns, 1e-3, 1e-3,
+                                     const_stride=1, threads=ns)
+    check(g, (y, steps, np.zeros(ns, np.int32), tot), f"mass_action exact N{N} m{method}")
+
+
 def test_legacy_symbol_reference_fixture():
     """The drop-in symbol, loaded as WIND's harness loads it, on the reference's
     own smoke-test fixture (cuda/debug/inputKatz96_1_1_1.h, NUM_CONST = 2),

@@ -20,7 +20,7 @@ MODE_EXACT, MODE_FAST = 0, 1
 KERNEL_AUTO, KERNEL_BLOCK, KERNEL_LANE = 0, 1, 2
 CTRL_REFERENCE, CTRL_ADAPTIVE = 0, 1
 SOLVER_AUTO, SOLVER_VALU, SOLVER_MFMA_F32, SOLVER_MFMA_BF16, SOLVER_MFMA_BF16X3 = 0, 1, 2, 3, 4
-FLAG_GRAPH, FLAG_NO_AUTOTUNE = 1, 2
+FLAG_GRAPH, FLAG_NO_AUTOTUNE, FLAG_PIVOT = 1, 2, 4
 NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION = 0, 1, 2, 3
 STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 
@@ -41,7 +41,13 @@ class WindOptions(C.Structure):
         ("steps_out", C.c_void_p), ("status_out", C.c_void_p), ("total_steps", C.c_void_p),
         ("stream", C.c_void_p), ("kernel_ms", C.c_void_p), ("controller", C.c_int),
         ("solver", C.c_int), ("ir_sweeps", C.c_int), ("flags", C.c_int), ("cost_hint", C.c_void_p),
+        ("trace", C.c_void_p),
     ]
+
+
+class WindTrace(C.Structure):
+    _fields_ = [("capacity", C.c_int), ("h", C.c_void_p), ("accept", C.c_void_p), ("count", C.c_void_p),
+                ("y_start", C.c_void_p), ("y1", C.c_void_p), ("y2", C.c_void_p), ("flag", C.c_void_p)]
 
 
 class WindRequestDesc(C.Structure):
@@ -191,10 +197,13 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
               absolute, relative, method=SIE, mode=MODE_EXACT, kernel=KERNEL_AUTO,
               const_stride=0, device_pointers=False, max_attempts=0, steps_out=None,
               status_out=None, stream=None, want_kernel_ms=False, controller=CTRL_REFERENCE,
-              solver=SOLVER_AUTO, ir_sweeps=0, cost_hint=None):
+              solver=SOLVER_AUTO, ir_sweeps=0, cost_hint=None, trace=None, flags=0):
     """Integrates `nsys` systems in place.  Returns (total_substeps, kernel_ms).
     `cost_hint`: optional int32 per-system cost estimates (e.g. a previous
-    call's steps_out); lane kernels start systems in decreasing order of it."""
+    call's steps_out); lane kernels start systems in decreasing order of it.
+    `trace`: optional dict for wind_trace: capacity, and arrays h, accept,
+    count, y_start (replay) and y1, y2, flag (record), host or device like the
+    state."""
     lib = load()
     o = WindOptions()
     lib.wind_options_init(C.byref(o))
@@ -204,6 +213,7 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
     o.solver = solver
     o.ir_sweeps = ir_sweeps
     o.max_attempts = max_attempts
+    o.flags = flags
     total = C.c_int64(0)
     kms = C.c_float(0.0)
     o.total_steps = C.cast(C.byref(total), C.c_void_p)
@@ -213,6 +223,11 @@ def integrate(net: Network, tnow, tend, n_integration_steps, constants, equation
     o.status_out = _ptr(status_out)
     o.cost_hint = _ptr(cost_hint)
     o.stream = stream
+    tr = None
+    if trace is not None:
+        tr = WindTrace(int(trace["capacity"]), *(_ptr(trace.get(k)) for k in ("h", "accept", "count", "y_start",
+                                                                             "y1", "y2", "flag")))
+        o.trace = C.cast(C.byref(tr), C.c_void_p)
     if not device_pointers:
         for a, dt in ((constants, np.float32), (equations, np.float32)):
             if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags.c_contiguous):

@@ -540,9 +540,13 @@ __global__ void __launch_bounds__(512) ma_dense_integrate(IntegrateArgs a, MassA
         float tnow = a.tnow, h = a.h0;
         int unsolved = 0, attempts = 0, status = WIND_STATUS_OK;
         const bool replay = TRACE && a.trace.h != nullptr;
-        const int rcount = replay ? a.trace.count[sid] : 0;
+        const int rcount = replay ? min(a.trace.count[sid], a.trace.capacity) : 0;
         while (replay ? attempts < rcount : tnow < a.tend) {
             const size_t slot = TRACE ? (size_t)sid * a.trace.capacity + attempts : 0;
+            if (replay && a.trace.y_start) {
+                for (int i = t; i < N; i += T) L.y[i] = a.trace.y_start[slot * N + i];
+                __syncthreads();
+            }
             const float hh = replay ? a.trace.h[slot] : fminf(a.tend - tnow, h);
             const float tb = tnow + hh;
             const float hs_full = tb - tnow, hs_half = (tb - tnow) / 2.0f;

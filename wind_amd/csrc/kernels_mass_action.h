@@ -114,14 +114,59 @@ __device__ inline void block_gje_exact(int N, MALayout &L) {
     __syncthreads();
 }
 
-// Fast LU solve of A x = f, A[i][j] = M[j*N + i]; x in L.g.
+// Row of the largest |A(i, k)|, i >= k (the first of equals), searched by
+// wave 0: each lane scans rows k + lane, k + lane + 64, ..., then the wave
+// reduces (value, row) pairs with shuffles.  Written to *piv.
+__device__ inline void pivot_search(const float *M, int N, int k, int *piv) {
+    if (threadIdx.x >= 64) return;
+    float best = -1.0f;
+    int row = k;
+    for (int i = k + (int)threadIdx.x; i < N; i += 64) {
+        const float v = fabsf(M[k * N + i]);
+        if (v > best) {
+            best = v;
+            row = i;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const int orow = __shfl_xor(row, off);
+        if (ob > best || (ob == best && orow < row)) {
+            best = ob;
+            row = orow;
+        }
+    }
+    if (threadIdx.x == 0) *piv = row;
+}
+
+// Fast LU solve of A x = f, A[i][j] = M[j*N + i]; x in L.g.  Unpivoted like
+// the reference's Gauss-Jordan, or (PIVOT) with partial pivoting: before
+// eliminating column k, rows k and the pivot_search row exchange their
+// entries in columns k.. and in the right-hand side.
+template <bool PIVOT = false>
 __device__ inline void block_lu_fast(int N, MALayout &L) {
     const int T = blockDim.x, t = threadIdx.x;
     float *M = L.M;
     float *b = L.g;
+    __shared__ int s_piv;
     for (int i = t; i < N; i += T) b[i] = L.f[i];
     __syncthreads();
     for (int k = 0; k < N; k++) {
+        if constexpr (PIVOT) {
+            pivot_search(M, N, k, &s_piv);
+            __syncthreads();
+            const int p = s_piv;
+            if (p != k) {
+                for (int j = k + t; j <= N; j += T) {
+                    float *a = j < N ? M + (size_t)j * N : b;
+                    const float v = a[k];
+                    a[k] = a[p];
+                    a[p] = v;
+                }
+                __syncthreads();
+            }
+        }
         const float ip = __builtin_amdgcn_rcpf(M[k * N + k]);
         for (int i = k + 1 + t; i < N; i += T) M[k * N + i] = M[k * N + i] * ip;  // l_i
         __syncthreads();
@@ -148,29 +193,21 @@ __device__ inline void block_lu_fast(int N, MALayout &L) {
     }
 }
 
-template <int METHOD, bool FAST>
-__global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassActionDevice net,
-                                                          float *gscratch, int mat_in_lds) {
-    extern __shared__ float lds[];
-    const int N = net.N, R = net.R, t = threadIdx.x, T = blockDim.x;
-    MALayout L;
-    L.y = lds;
-    L.yw = L.y + N;
-    L.y1 = L.yw + N;
-    L.f = L.y1 + N;
-    L.g = L.f + N;
-    L.w = L.g + N;
-    L.da = L.w + R;
-    L.db = L.da + R;
-    float *mats = mat_in_lds ? (L.db + R) : gscratch + (size_t)blockIdx.x * 2 * N * N;
-    L.M = mats;
-    L.Inv = mats + N * N;
+// The per-system loop of the one-workgroup-per-system kernels: each
+// workgroup takes systems blockIdx.x, blockIdx.x + gridDim.x, ... and runs
+// the step-doubling attempts of common_gold.c:77-148 with the solver of the
+// mode.  `ev` evaluates the network: ev.load(a, sid) reads the system's
+// constants, ev.eval(yv, L, nh, want_jac) writes f(yv) to L.f and, with
+// want_jac, L.M = I + nh J(yv) (column-major), ending with a barrier.
+template <int METHOD, bool FAST, class Eval, bool PIVOT = false>
+__device__ inline void block_integrate_systems(const IntegrateArgs &a, int N, MALayout &L, Eval &ev) {
+    const int t = threadIdx.x, T = blockDim.x;
     __shared__ int s_flag;
     __shared__ double s_err[4];
     unsigned long long steps = 0;
     for (int sid = blockIdx.x; sid < a.nsys; sid += gridDim.x) {
         for (int i = t; i < N; i += T) L.y[i] = a.equations[(size_t)sid * N + i];
-        const float scale = a.constants[(size_t)sid * a.const_stride];
+        ev.load(a, sid);
         __syncthreads();
         float tnow = a.tnow, h = a.h0;
         int unsolved = 0, attempts = 0, status = WIND_STATUS_OK;
@@ -184,10 +221,10 @@ __global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassA
                     for (int i = t; i < N; i += T) L.yw[i] = L.y[i];
                     __syncthreads();
                 }
-                ma_eval(net, L.yw, scale, L, -hs, METHOD == WIND_SIE);
+                ev.eval(L.yw, L, -hs, METHOD == WIND_SIE);
                 if (METHOD == WIND_SIE) {
                     if (FAST)
-                        block_lu_fast(N, L);
+                        block_lu_fast<PIVOT>(N, L);
                     else
                         block_gje_exact(N, L);
                     for (int i = t; i < N; i += T)
@@ -244,6 +281,38 @@ __global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassA
         __syncthreads();
     }
     if (t == 0 && steps) atomicAdd(a.counters + 1, steps);
+}
+
+// Network evaluation of the table-driven mass-action networks (one constant
+// per system: the rate scale).
+struct MassActionEval {
+    const MassActionDevice &net;
+    float scale;
+    __device__ void load(const IntegrateArgs &a, int sid) { scale = a.constants[(size_t)sid * a.const_stride]; }
+    __device__ void eval(const float *yv, MALayout &L, float nh, bool want_jac) {
+        ma_eval(net, yv, scale, L, nh, want_jac);
+    }
+};
+
+template <int METHOD, bool FAST, bool PIVOT = false>
+__global__ void __launch_bounds__(256) ma_block_integrate(IntegrateArgs a, MassActionDevice net,
+                                                          float *gscratch, int mat_in_lds) {
+    extern __shared__ float lds[];
+    const int N = net.N, R = net.R;
+    MALayout L;
+    L.y = lds;
+    L.yw = L.y + N;
+    L.y1 = L.yw + N;
+    L.f = L.y1 + N;
+    L.g = L.f + N;
+    L.w = L.g + N;
+    L.da = L.w + R;
+    L.db = L.da + R;
+    float *mats = mat_in_lds ? (L.db + R) : gscratch + (size_t)blockIdx.x * 2 * N * N;
+    L.M = mats;
+    L.Inv = mats + N * N;
+    MassActionEval ev{net, 0.0f};
+    block_integrate_systems<METHOD, FAST, MassActionEval, PIVOT>(a, N, L, ev);
 }
 
 }  // namespace wind

@@ -75,15 +75,43 @@ __device__ inline void gje_solve_exact(float (&Mx)[N * N], const float (&f)[N], 
 }
 
 // Fast-mode solve of A x = f with A[i][j] = Ms[j*N + i] (the stored matrix
-// is column-major): unpivoted LU elimination, fused multiply-adds and one
-// reciprocal per pivot.  About N^3/3 FMAs against the inverse's N^3.
-template <int N>
+// is column-major): LU elimination, fused multiply-adds and one reciprocal
+// per pivot.  About N^3/3 FMAs against the inverse's N^3.  Unpivoted like the
+// reference's Gauss-Jordan, or (PIVOT) with partial pivoting: at column k the
+// row i >= k with the largest |A(i, k)| (the first of equals) is exchanged
+// with row k, by selects, since the registers cannot be indexed at run time.
+template <int N, bool PIVOT = false>
 __device__ inline void lu_solve_fast(float (&Ms)[N * N], const float (&f)[N], float (&x)[N]) {
     float b[N], ip[N];
 #pragma unroll
     for (int i = 0; i < N; i++) b[i] = f[i];
 #pragma unroll
     for (int k = 0; k < N; k++) {
+        if constexpr (PIVOT) {
+            float best = fabsf(Ms[k * N + k]);
+            int p = k;
+#pragma unroll
+            for (int i = k + 1; i < N; i++) {
+                const float v = fabsf(Ms[k * N + i]);
+                if (v > best) {
+                    best = v;
+                    p = i;
+                }
+            }
+#pragma unroll
+            for (int i = k + 1; i < N; i++) {
+                const bool sw = p == i;
+#pragma unroll
+                for (int j = k; j < N; j++) {
+                    const float t = Ms[j * N + k], u = Ms[j * N + i];
+                    Ms[j * N + k] = sw ? u : t;
+                    Ms[j * N + i] = sw ? t : u;
+                }
+                const float t = b[k], u = b[i];
+                b[k] = sw ? u : t;
+                b[i] = sw ? t : u;
+            }
+        }
         ip[k] = __builtin_amdgcn_rcpf(Ms[k * N + k]);
 #pragma unroll
         for (int i = k + 1; i < N; i++) {
@@ -109,7 +137,7 @@ __device__ inline void lu_solve_fast(float (&Ms)[N * N], const float (&f)[N], fl
 // are -0, every multiplier is +-0 and, for finite J and f, z = +-0 and y
 // stays as it is (a non-finite f or J gives NaN, as y + 0 (I - 0 J)^-1 f
 // does).  The factorisation overwrites J.
-template <int N>
+template <int N, bool PIVOT = false>
 __device__ inline void sie_update_fast(float (&J)[N * N], const float (&f)[N], float hs, const float (&y)[N],
                                        float (&out)[N]) {
     const float c = __builtin_amdgcn_rcpf(hs);
@@ -118,7 +146,7 @@ __device__ inline void sie_update_fast(float (&J)[N * N], const float (&f)[N], f
     float nf[N], z[N];
 #pragma unroll
     for (int i = 0; i < N; i++) nf[i] = -f[i];
-    lu_solve_fast<N>(J, nf, z);
+    lu_solve_fast<N, PIVOT>(J, nf, z);
 #pragma unroll
     for (int i = 0; i < N; i++) out[i] = y[i] + z[i];
 }
@@ -138,7 +166,7 @@ __device__ inline void substep(const typename Net::Params &p, const float *c, fl
         Net::template jac<M>(p, c, y, A);
         if constexpr (M::kFast) {
             float yn[N];
-            sie_update_fast<N>(A, f, hs, y, yn);
+            sie_update_fast<N, M::kPivot>(A, f, hs, y, yn);
 #pragma unroll
             for (int i = 0; i < N; i++) y[i] = yn[i];
         } else {
@@ -177,8 +205,8 @@ __device__ inline void substep_pair(const typename Net::Params &p, const float *
             float A[N * N];
 #pragma unroll
             for (int e = 0; e < N * N; e++) A[e] = J[e];
-            sie_update_fast<N>(A, f, hs1, y, y1);
-            sie_update_fast<N>(J, f, hs2, y, y2);
+            sie_update_fast<N, M::kPivot>(A, f, hs1, y, y1);
+            sie_update_fast<N, M::kPivot>(J, f, hs2, y, y2);
             return;
         }
 #pragma unroll
@@ -234,8 +262,11 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
     bool replay = false;
     if constexpr (TRACE) {
         replay = a.trace.h != nullptr;
-        if (replay && s.attempts >= a.trace.count[sid]) return true;
+        if (replay && s.attempts >= min(a.trace.count[sid], a.trace.capacity)) return true;
         slot = (size_t)sid * a.trace.capacity + s.attempts;
+        if (replay && a.trace.y_start)
+#pragma unroll
+            for (int i = 0; i < N; i++) s.y[i] = a.trace.y_start[slot * N + i];
     }
     const float hh = replay ? a.trace.h[slot] : fminf(a.tend - s.tnow, s.h);
     const float tb = s.tnow + hh;
@@ -273,7 +304,7 @@ __device__ inline bool attempt(const typename Net::Params &p, const IntegrateArg
 #pragma unroll
                 for (int i = 0; i < N; i++) s.y[i] = extrap ? 2.0f * y2[i] - y1[i] : y2[i];
             }
-            return s.attempts >= a.trace.count[sid];
+            return s.attempts >= min(a.trace.count[sid], a.trace.capacity);
         }
     }
     s.attempts++;

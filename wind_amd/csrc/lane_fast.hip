@@ -17,17 +17,18 @@ using namespace wind;
 
 namespace {
 
-template <class Net, int METHOD>
+// M: MathFast, or MathFastPivot for partial pivoting in the LU solves.
+template <class Net, int METHOD, class M>
 __global__ void __launch_bounds__(256, WIND_LANE_FAST_MIN_BLOCKS)
     lane_fast_kernel(IntegrateArgs a, typename Net::Params p) {
-    lane_integrate_body<Net, METHOD, MathFast>(a, p);
+    lane_integrate_body<Net, METHOD, M>(a, p);
 }
 
 // The same integration with a per-attempt trace or replay (a.trace); the
 // arithmetic of each attempt is lane_fast_kernel's.
-template <class Net, int METHOD>
+template <class Net, int METHOD, class M>
 __global__ void __launch_bounds__(256) lane_fast_trace_kernel(IntegrateArgs a, typename Net::Params p) {
-    lane_integrate_body<Net, METHOD, MathFast, true>(a, p);
+    lane_integrate_body<Net, METHOD, M, true>(a, p);
 }
 
 // Start-order estimate from the initial state alone: x = (tend - tnow) *
@@ -106,7 +107,10 @@ constexpr int kLaneFastBlocksPerCU = 2;
 template <class Net, int METHOD>
 int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
-    auto k = a.trace.capacity > 0 ? lane_fast_trace_kernel<Net, METHOD> : lane_fast_kernel<Net, METHOD>;
+    auto k = a.pivot ? (a.trace.capacity > 0 ? lane_fast_trace_kernel<Net, METHOD, MathFastPivot>
+                                             : lane_fast_kernel<Net, METHOD, MathFastPivot>)
+                     : (a.trace.capacity > 0 ? lane_fast_trace_kernel<Net, METHOD, MathFast>
+                                             : lane_fast_kernel<Net, METHOD, MathFast>);
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));

@@ -100,14 +100,16 @@ struct MassActionPlan {
     size_t lds = 0;
 };
 
-static MassActionPlan mass_action_plan(const MassActionDevice &d, int mode) {
+// PIVOT (fast mode, WIND_FLAG_PIVOT) takes the LDS LU with partial pivoting
+// instead of the dense solver, whose diagonal blocks are unpivoted.
+static MassActionPlan mass_action_plan(const MassActionDevice &d, int mode, bool pivot = false) {
     MassActionPlan p;
     int dev = 0, max_lds = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     const int N = d.N;
     const bool fast = mode == WIND_MODE_FAST;
-    if (fast && (N + 15) / 16 * 16 <= 192) {
+    if (fast && !pivot && (N + 15) / 16 * 16 <= 192) {
         const size_t lds = dense_lds_bytes(N, d.R);
         if (lds + 128 <= (size_t)max_lds) {
             p.dense = true;
@@ -137,6 +139,12 @@ struct BlockShape {
 };
 
 template <int METHOD>
+auto block_kernel(bool fast, bool pivot) {
+    return fast ? (pivot ? ma_block_integrate<METHOD, true, true> : ma_block_integrate<METHOD, true>)
+                : ma_block_integrate<METHOD, false>;
+}
+
+template <int METHOD>
 int block_shape(int num_cu, const MassActionDevice &d, const MassActionPlan &plan, bool fast, const SolverChoice &sc,
                 int nsys, BlockShape *s) {
     s->T = d.N <= 64 ? 64 : 256;
@@ -146,7 +154,7 @@ int block_shape(int num_cu, const MassActionDevice &d, const MassActionPlan &pla
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (s->lds + 64 > (size_t)max_lds) return wind_fail(WIND_ERR_UNSUPPORTED, "network too large for LDS");
-    auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
+    auto kfn = block_kernel<METHOD>(fast, sc.pivot != 0);
     if (s->lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)s->lds);
@@ -170,7 +178,7 @@ int mass_action_scratch_bytes(int num_cu, const wind_network *net, int mode, con
                               size_t *bytes) {
     *bytes = 0;
     if (net->builtin != WIND_NET_MASS_ACTION) return WIND_OK;
-    const MassActionPlan plan = mass_action_plan(net->ma, mode);
+    const MassActionPlan plan = mass_action_plan(net->ma, mode, sc.pivot != 0);
     if (plan.dense || plan.in_lds) return WIND_OK;
     BlockShape s;
     const int rc = block_shape<METHOD>(num_cu, net->ma, plan, mode == WIND_MODE_FAST, sc, nsys, &s);
@@ -202,7 +210,7 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
     const MassActionDevice &d = net->ma;
     const int N = d.N;
     const bool fast = mode == WIND_MODE_FAST;
-    const MassActionPlan plan = mass_action_plan(d, mode);
+    const MassActionPlan plan = mass_action_plan(d, mode, sc.pivot != 0);
     if (plan.dense) {
         const size_t lds = plan.lds;
         int T = ((N + 15) & ~15) <= 64 ? 64 : 512;
@@ -218,6 +226,9 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
             return launch_dense<METHOD, PREC_MFMA_BF16X3>(num_cu, d, a, sc.ir_sweeps, lds, T, sc, st);
         }
     }
+    if (a.trace.capacity > 0)
+        return wind_fail(WIND_ERR_UNSUPPORTED, "trace: mass-action networks are traced by the dense solver "
+                                               "(fast mode, N <= 176) only");
     BlockShape s;
     int rc = block_shape<METHOD>(num_cu, d, plan, fast, sc, a.nsys, &s);
     if (rc) return rc;
@@ -231,7 +242,7 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
         if (rc) return rc;
         scratch = g_scratch.ptr;
     }
-    auto kfn = fast ? ma_block_integrate<METHOD, true> : ma_block_integrate<METHOD, false>;
+    auto kfn = block_kernel<METHOD>(fast, sc.pivot != 0);
     hipLaunchKernelGGL(kfn, dim3(s.grid), dim3(s.T), s.lds, st, a, d, scratch, s.in_lds ? 1 : 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));

@@ -51,10 +51,10 @@ struct Pool {
 // context lock.
 struct TuneKey {
     const wind_network *net;
-    int method, mode, controller, solver, ir;
+    int method, mode, controller, solver, ir, pivot;
     bool operator<(const TuneKey &o) const {
-        return std::tie(net, method, mode, controller, solver, ir) <
-               std::tie(o.net, o.method, o.mode, o.controller, o.solver, o.ir);
+        return std::tie(net, method, mode, controller, solver, ir, pivot) <
+               std::tie(o.net, o.method, o.mode, o.controller, o.solver, o.ir, o.pivot);
     }
 };
 struct TuneVal {
@@ -63,6 +63,19 @@ struct TuneVal {
 };
 std::map<TuneKey, TuneVal> g_tuned;
 
+// What one wind_integrate call uses on the device: its stream, timing
+// events and pooled buffers (grow-only).  Calls take a free slot, so that
+// callers on several threads run concurrently.
+struct CallSlot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Pool constants, equations, steps, status, counters;
+    Pool hint, order;  // cost hints copied from the host; start order and class counts
+    Pool activity;     // initial-state work estimates (start order without caller hints)
+    Pool trace;        // per-attempt trace arrays of host-pointer calls
+    bool busy = false;
+};
+
 struct Context {
     std::mutex mu;
     bool ready = false;
@@ -70,9 +83,9 @@ struct Context {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    Pool constants, equations, steps, status, counters;
-    Pool hint, order;  // cost hints copied from the host; start order and class counts
-    Pool activity;     // initial-state work estimates (start order without caller hints)
+    std::vector<CallSlot *> slots;  // wind_integrate call slots (never freed)
+    Pool constants, counters;       // batches and the output-step driver
+    Pool hint, order;
     Pool out_steps;    // output-step driver: per-system sub-steps of the last interval
     // batch execution
     std::vector<hipStream_t> bucket_streams;
@@ -155,6 +168,12 @@ Context &ctx() {
     return c;
 }
 
+// Launches whose matrices exceed LDS share the process-wide matrix scratch
+// (mass_action_scratch_reserve): a call that uses it holds this lock from
+// the reserve through its stream synchronisation.  Lock order: the context
+// lock first.
+std::mutex g_scratch_mu;
+
 int ensure_ready(Context &c) {
     if (c.ready) return WIND_OK;
     WIND_HIP(hipGetDevice(&c.device));
@@ -167,6 +186,35 @@ int ensure_ready(Context &c) {
     c.ready = true;
     return WIND_OK;
 }
+
+// A free call slot (created on first need); caller holds the context lock.
+int acquire_slot(Context &c, CallSlot **out) {
+    for (CallSlot *sl : c.slots)
+        if (!sl->busy) {
+            sl->busy = true;
+            *out = sl;
+            return WIND_OK;
+        }
+    CallSlot *sl = new CallSlot();
+    if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&sl->ev0) != hipSuccess || hipEventCreate(&sl->ev1) != hipSuccess) {
+        delete sl;
+        return fail(WIND_ERR_DEVICE, "cannot create a stream or events for a call slot");
+    }
+    sl->busy = true;
+    c.slots.push_back(sl);
+    *out = sl;
+    return WIND_OK;
+}
+
+struct SlotGuard {
+    Context &c;
+    CallSlot *sl;
+    ~SlotGuard() {
+        std::lock_guard<std::mutex> lock(c.mu);
+        sl->busy = false;
+    }
+};
 
 int reserve(Pool &p, size_t bytes) {
     if (bytes <= p.bytes) return WIND_OK;
@@ -234,10 +282,12 @@ int launch_small(int num_cu, const wind_network *net, const IntegrateArgs &a, in
     if (kernel == WIND_KERNEL_BLOCK) {
         if (mode != WIND_MODE_EXACT)
             return fail(WIND_ERR_UNSUPPORTED, "the block kernel implements exact mode only");
+        if (a.trace.capacity > 0) return fail(WIND_ERR_UNSUPPORTED, "trace: the block kernel has no trace");
         const int grid = scaled_grid(num_cu, 32, sc, a.nsys);
         hipLaunchKernelGGL((block_integrate<Net, METHOD>), dim3(grid), dim3(64), 0, st, a, p);
     } else if (mode == WIND_MODE_EXACT) {
-        auto k = lane_integrate<Net, METHOD, MathExact>;
+        auto k = a.trace.capacity > 0 ? lane_integrate<Net, METHOD, MathExact, true>
+                                      : lane_integrate<Net, METHOD, MathExact>;
         int grid = 0;
         int rc = persistent_grid(num_cu, k, 256, a.nsys, &grid, sc);
         if (rc) return rc;
@@ -250,20 +300,57 @@ int launch_small(int num_cu, const wind_network *net, const IntegrateArgs &a, in
 }
 
 template <int METHOD>
-int dispatch(int num_cu, const wind_network *net, const IntegrateArgs &a, int mode,
+int dispatch(int num_cu, const wind_network *net, const IntegrateArgs &a_in, int mode,
              int kernel, const SolverChoice &sc, hipStream_t st) {
+    IntegrateArgs a = a_in;
+    a.pivot = sc.pivot;
     switch (net->builtin) {
     case WIND_NET_NR_TEST: return launch_small<NetNRTest, METHOD>(num_cu, net, a, mode, kernel, sc, st);
     case WIND_NET_KATZ96: return launch_small<NetKatz96, METHOD>(num_cu, net, a, mode, kernel, sc, st);
     case WIND_NET_HHE6: return launch_small<NetHHe6, METHOD>(num_cu, net, a, mode, kernel, sc, st);
     case WIND_NET_MASS_ACTION: return launch_mass_action<METHOD>(num_cu, net, a, mode, kernel, sc, st);
     case WIND_NET_PLUGIN: {
-        const int rc = net->plugin->launch(&a, METHOD, mode, kernel, num_cu, sc.share, sc.grid, st);
-        if (rc == WIND_ERR_UNSUPPORTED) return fail(rc, "generated networks: the block kernel is exact mode only");
+        if (a.trace.capacity > 0) return fail(WIND_ERR_UNSUPPORTED, "trace: generated networks have no trace");
+        // Matrices beyond LDS live in the process-wide scratch (callers hold
+        // the context lock; wind_launch_async and the allocation-free entry
+        // point refuse such launches).
+        size_t need = 0;
+        int rc = net->plugin->scratch_bytes(a.nsys, METHOD, mode, kernel, num_cu, sc.share, sc.grid, &need);
+        if (rc) return fail(rc, "generated network: no launch shape");
+        float *scratch = nullptr;
+        if (need) {
+            rc = mass_action_scratch_reserve(need);
+            if (rc) return rc;
+            scratch = (float *)mass_action_scratch_ptr();
+        }
+        rc = net->plugin->launch(&a, METHOD, mode, kernel, num_cu, sc.share, sc.grid, scratch, need, st);
+        if (rc == WIND_ERR_UNSUPPORTED)
+            return fail(rc, "generated network: no kernel for this mode and kernel family at N = " +
+                                std::to_string(net->N) + " (lane kernels: N <= 16; 64-thread block kernel: exact "
+                                "mode, N <= 64)");
         return rc ? fail(rc, "generated network: kernel launch failed") : WIND_OK;
     }
     default: return fail(WIND_ERR_UNSUPPORTED, "unknown network");
     }
+}
+
+// Device scratch bytes a launch needs (matrices beyond LDS), 0 if none.
+template <int METHOD>
+int launch_scratch_bytes(int num_cu, const wind_network *net, int mode, int kernel, const SolverChoice &sc, int nsys,
+                         size_t *bytes) {
+    *bytes = 0;
+    if (net->builtin == WIND_NET_MASS_ACTION) return mass_action_scratch_bytes<METHOD>(num_cu, net, mode, sc, nsys, bytes);
+    if (net->builtin == WIND_NET_PLUGIN) {
+        const int rc = net->plugin->scratch_bytes(nsys, METHOD, mode, kernel, num_cu, sc.share, sc.grid, bytes);
+        return rc ? fail(rc, "generated network: no launch shape") : WIND_OK;
+    }
+    return WIND_OK;
+}
+
+int scratch_bytes_for(int num_cu, const wind_network *net, int method, int mode, int kernel, const SolverChoice &sc,
+                      int nsys, size_t *bytes) {
+    return method == WIND_SIE ? launch_scratch_bytes<WIND_SIE>(num_cu, net, mode, kernel, sc, nsys, bytes)
+                              : launch_scratch_bytes<WIND_RK2>(num_cu, net, mode, kernel, sc, nsys, bytes);
 }
 
 }  // namespace
@@ -286,12 +373,88 @@ static int check_integrate_args(const wind_network *net, const wind_options &opt
     if (opts.solver < WIND_SOLVER_AUTO || opts.solver > WIND_SOLVER_MFMA_BF16X3)
         return fail(WIND_ERR_ARGUMENT, "unknown solver");
     if (opts.ir_sweeps < 0 || opts.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
+    if ((opts.flags & WIND_FLAG_PIVOT) && opts.mode != WIND_MODE_FAST)
+        return fail(WIND_ERR_ARGUMENT, "WIND_FLAG_PIVOT applies to fast mode (exact mode is the reference's "
+                                       "unpivoted Gauss-Jordan)");
     if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
     if (Neqn != net->N)
         return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network (N = " +
                                            std::to_string(net->N) + ")");
     if (n_integration_steps < 0)
         return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0 (negative initial step)");
+    return WIND_OK;
+}
+
+// Device-side per-attempt trace of a wind_integrate call: checks the
+// wind_trace, and for host pointers stages it through pool `pool` (replay
+// arrays copied in on `st`).  Outputs to copy back afterwards are listed in
+// *back (device source, host destination, bytes).
+struct TraceCopy {
+    const void *src;
+    void *dst;
+    size_t bytes;
+};
+static int setup_trace(Pool &pool, const wind_options &o, int n, int N, TraceArgs *tr, std::vector<TraceCopy> *back,
+                       hipStream_t st) {
+    const wind_trace &t = *o.trace;
+    if (t.capacity <= 0) return fail(WIND_ERR_ARGUMENT, "trace: capacity must be positive");
+    const bool replay = t.h || t.accept || t.count;
+    if (replay && !(t.h && t.accept && t.count)) return fail(WIND_ERR_ARGUMENT, "trace: replay needs h, accept and count");
+    if (!replay && !t.y1 && !t.y2 && !t.flag) return fail(WIND_ERR_ARGUMENT, "trace: nothing to replay or record");
+    const size_t slots = (size_t)n * t.capacity;
+    tr->capacity = t.capacity;
+    if (t.y_start && !replay) return fail(WIND_ERR_ARGUMENT, "trace: y_start needs a replay (h, accept, count)");
+    if (o.device_pointers) {
+        tr->h = t.h;
+        tr->accept = t.accept;
+        tr->count = t.count;
+        tr->y_start = t.y_start;
+        tr->y1 = t.y1;
+        tr->y2 = t.y2;
+        tr->flag = t.flag;
+        return WIND_OK;
+    }
+    if (replay)
+        for (int s = 0; s < n; s++)
+            if (t.count[s] < 0 || t.count[s] > t.capacity)
+                return fail(WIND_ERR_ARGUMENT, "trace: count[" + std::to_string(s) + "] outside [0, capacity]");
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bh = replay ? up(slots * 4) : 0, bc = replay ? up((size_t)n * 4) : 0;
+    const size_t by = up(slots * N * 4), bf = up(slots * 4);
+    const size_t total = 2 * bh + bc + (t.y_start ? by : 0) + (t.y1 ? by : 0) + (t.y2 ? by : 0) + (t.flag ? bf : 0);
+    int rc = reserve(pool, total);
+    if (rc) return rc;
+    char *p = (char *)pool.ptr;
+    if (replay) {
+        WIND_HIP(hipMemcpyAsync(p, t.h, slots * 4, hipMemcpyHostToDevice, st));
+        tr->h = (const float *)p;
+        p += bh;
+        WIND_HIP(hipMemcpyAsync(p, t.accept, slots * 4, hipMemcpyHostToDevice, st));
+        tr->accept = (const int32_t *)p;
+        p += bh;
+        WIND_HIP(hipMemcpyAsync(p, t.count, (size_t)n * 4, hipMemcpyHostToDevice, st));
+        tr->count = (const int32_t *)p;
+        p += bc;
+        if (t.y_start) {
+            WIND_HIP(hipMemcpyAsync(p, t.y_start, slots * N * 4, hipMemcpyHostToDevice, st));
+            tr->y_start = (const float *)p;
+            p += by;
+        }
+    }
+    if (t.y1) {
+        tr->y1 = (float *)p;
+        back->push_back({p, t.y1, slots * N * 4});
+        p += by;
+    }
+    if (t.y2) {
+        tr->y2 = (float *)p;
+        back->push_back({p, t.y2, slots * N * 4});
+        p += by;
+    }
+    if (t.flag) {
+        tr->flag = (int32_t *)p;
+        back->push_back({p, t.flag, slots * 4});
+    }
     return WIND_OK;
 }
 
@@ -447,6 +610,9 @@ int wind_integrate_outputs(const wind_network *net, double tnow, double tend, in
         return fail(WIND_ERR_ARGUMENT, "unknown solver");
     if (opts.ir_sweeps < 0 || opts.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
     if (n_output_steps < 1) return fail(WIND_ERR_ARGUMENT, "n_output_steps < 1");
+    if (opts.trace) return fail(WIND_ERR_ARGUMENT, "trace: supported by wind_integrate only");
+    if ((opts.flags & WIND_FLAG_PIVOT) && opts.mode != WIND_MODE_FAST)
+        return fail(WIND_ERR_ARGUMENT, "WIND_FLAG_PIVOT applies to fast mode");
     if (n_integration_steps < 0) return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0");
     if (Nsystems < 0) return fail(WIND_ERR_ARGUMENT, "Nsystems < 0");
     if (Neqn != net->N) return fail(WIND_ERR_ARGUMENT, "Nequations_per_system does not match the network");
@@ -527,6 +693,7 @@ int wind_integrate_outputs(const wind_network *net, double tnow, double tend, in
     SolverChoice sc;
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
+    sc.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
     double tcur = tnow;
     WIND_HIP(hipEventRecord(c.out_events[0], st));
     for (int k = 1; k <= n_output_steps; k++) {
@@ -611,11 +778,28 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     if (Nsystems == 0 || !(tnow < tend)) return WIND_OK;  // no attempt is made (common_gold.c:77)
     if (!constants || !equations) return fail(WIND_ERR_ARGUMENT, "NULL state or constants");
 
+    // The call takes a slot of its own (stream, events, buffers): callers on
+    // several threads only wait for each other where they share the matrix
+    // scratch.
     Context &c = ctx();
-    std::lock_guard<std::mutex> lock(c.mu);
-    rc = ensure_ready(c);
+    CallSlot *sl = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(c.mu);
+        rc = ensure_ready(c);
+        if (!rc) rc = acquire_slot(c, &sl);
+        if (rc) return rc;
+    }
+    SlotGuard release{c, sl};
+    SolverChoice sc;
+    sc.solver = opts.solver;
+    sc.ir_sweeps = opts.ir_sweeps;
+    sc.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
+    size_t scratch_need = 0;
+    rc = scratch_bytes_for(c.num_cu, net, opts.method, opts.mode, opts.kernel, sc, Nsystems, &scratch_need);
     if (rc) return rc;
-    hipStream_t st = opts.stream ? (hipStream_t)opts.stream : c.stream;
+    std::unique_lock<std::mutex> scratch_lock(g_scratch_mu, std::defer_lock);
+    if (scratch_need) scratch_lock.lock();
+    hipStream_t st = opts.stream ? (hipStream_t)opts.stream : sl->stream;
 
     const size_t eq_bytes = (size_t)Nsystems * Neqn * sizeof(float);
     // Reads of constants reach (Nsystems - 1) * stride + nconst floats.
@@ -627,21 +811,21 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     int32_t *d_steps = opts.steps_out;
     int32_t *d_status = opts.status_out;
     if (!opts.device_pointers) {
-        rc = reserve(c.constants, c_bytes);
-        if (!rc) rc = reserve(c.equations, eq_bytes);
-        if (!rc && opts.steps_out) rc = reserve(c.steps, (size_t)Nsystems * 4);
-        if (!rc && opts.status_out) rc = reserve(c.status, (size_t)Nsystems * 4);
+        rc = reserve(sl->constants, c_bytes);
+        if (!rc) rc = reserve(sl->equations, eq_bytes);
+        if (!rc && opts.steps_out) rc = reserve(sl->steps, (size_t)Nsystems * 4);
+        if (!rc && opts.status_out) rc = reserve(sl->status, (size_t)Nsystems * 4);
         if (rc) return rc;
-        WIND_HIP(hipMemcpyAsync(c.constants.ptr, constants, c_bytes, hipMemcpyHostToDevice, st));
-        WIND_HIP(hipMemcpyAsync(c.equations.ptr, equations, eq_bytes, hipMemcpyHostToDevice, st));
-        d_const = (const float *)c.constants.ptr;
-        d_eq = (float *)c.equations.ptr;
-        d_steps = opts.steps_out ? (int32_t *)c.steps.ptr : nullptr;
-        d_status = opts.status_out ? (int32_t *)c.status.ptr : nullptr;
+        WIND_HIP(hipMemcpyAsync(sl->constants.ptr, constants, c_bytes, hipMemcpyHostToDevice, st));
+        WIND_HIP(hipMemcpyAsync(sl->equations.ptr, equations, eq_bytes, hipMemcpyHostToDevice, st));
+        d_const = (const float *)sl->constants.ptr;
+        d_eq = (float *)sl->equations.ptr;
+        d_steps = opts.steps_out ? (int32_t *)sl->steps.ptr : nullptr;
+        d_status = opts.status_out ? (int32_t *)sl->status.ptr : nullptr;
     }
-    rc = reserve(c.counters, 4 * sizeof(unsigned long long));
+    rc = reserve(sl->counters, 4 * sizeof(unsigned long long));
     if (rc) return rc;
-    WIND_HIP(hipMemsetAsync(c.counters.ptr, 0, 4 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)sl->counters.ptr, 4);
 
     IntegrateArgs a;
     a.tnow = tnow;
@@ -658,9 +842,9 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     a.steps_out = d_steps;
     a.status_out = d_status;
     a.index = nullptr;
-    a.counters = (unsigned long long *)c.counters.ptr;
+    a.counters = (unsigned long long *)sl->counters.ptr;
 
-    WIND_HIP(hipEventRecord(c.ev0, st));
+    WIND_HIP(hipEventRecord(sl->ev0, st));
     // Start order for the kernels that take systems from a queue: from the
     // caller's cost hints, or (fast mode, compiled-in networks, batches
     // larger than the device runs at once) from an estimate of each
@@ -668,32 +852,36 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     if (opts.cost_hint && queue_kernel(net, opts)) {
         const int32_t *d_hint = opts.cost_hint;
         if (!opts.device_pointers) {
-            rc = reserve(c.hint, (size_t)Nsystems * 4);
+            rc = reserve(sl->hint, (size_t)Nsystems * 4);
             if (rc) return rc;
-            WIND_HIP(hipMemcpyAsync(c.hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
-            d_hint = (const int32_t *)c.hint.ptr;
+            WIND_HIP(hipMemcpyAsync(sl->hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
+            d_hint = (const int32_t *)sl->hint.ptr;
         }
-        rc = order_by_hint(c.order, d_hint, Nsystems, st, &a.index);
+        rc = order_by_hint(sl->order, d_hint, Nsystems, st, &a.index);
         if (rc) return rc;
     } else if (auto_order(net, opts, Nsystems)) {
-        rc = reserve(c.hint, (size_t)Nsystems * 4);
-        if (!rc) rc = reserve(c.activity, ((size_t)Nsystems + 64) * 4);
+        rc = reserve(sl->hint, (size_t)Nsystems * 4);
+        if (!rc) rc = reserve(sl->activity, ((size_t)Nsystems + 64) * 4);
         if (rc) return rc;
-        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)c.activity.ptr, 32);
-        rc = activity_hints(c.num_cu, net, a, (int32_t *)c.hint.ptr, (unsigned *)c.activity.ptr, st);
-        if (!rc) rc = order_by_hint(c.order, (const int32_t *)c.hint.ptr, Nsystems, st, &a.index);
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)sl->activity.ptr, 32);
+        rc = activity_hints(c.num_cu, net, a, (int32_t *)sl->hint.ptr, (unsigned *)sl->activity.ptr, st);
+        if (!rc) rc = order_by_hint(sl->order, (const int32_t *)sl->hint.ptr, Nsystems, st, &a.index);
         if (rc) return rc;
     }
-    SolverChoice sc;
-    sc.solver = opts.solver;
-    sc.ir_sweeps = opts.ir_sweeps;
+    std::vector<TraceCopy> trace_back;
+    if (opts.trace) {
+        rc = setup_trace(sl->trace, opts, Nsystems, Neqn, &a.trace, &trace_back, st);
+        if (rc) return rc;
+    }
     rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(c.num_cu, net, a, opts.mode, opts.kernel, sc, st)
                                  : dispatch<WIND_RK2>(c.num_cu, net, a, opts.mode, opts.kernel, sc, st);
     if (rc) return rc;
-    WIND_HIP(hipEventRecord(c.ev1, st));
+    WIND_HIP(hipEventRecord(sl->ev1, st));
 
     unsigned long long total = 0;
     WIND_HIP(hipMemcpyAsync(&total, a.counters + 1, sizeof(total), hipMemcpyDeviceToHost, st));
+    for (const TraceCopy &tc : trace_back)
+        WIND_HIP(hipMemcpyAsync(tc.dst, tc.src, tc.bytes, hipMemcpyDeviceToHost, st));
     if (!opts.device_pointers) {
         WIND_HIP(hipMemcpyAsync(equations, d_eq, eq_bytes, hipMemcpyDeviceToHost, st));
         if (opts.steps_out)
@@ -704,7 +892,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
                                     hipMemcpyDeviceToHost, st));
     }
     WIND_HIP(hipStreamSynchronize(st));
-    if (opts.kernel_ms) WIND_HIP(hipEventElapsedTime(opts.kernel_ms, c.ev0, c.ev1));
+    if (opts.kernel_ms) WIND_HIP(hipEventElapsedTime(opts.kernel_ms, sl->ev0, sl->ev1));
     if (opts.total_steps) *opts.total_steps = (int64_t)total;
     return WIND_OK;
 }
@@ -782,9 +970,7 @@ extern "C" int wind_integrate_noalloc(const wind_network *net, float tnow, float
         return fail(WIND_ERR_ARGUMENT, "wind_integrate_noalloc: Nsystems " + std::to_string(Nsystems) +
                                            " exceeds its scratch maximum of " +
                                            std::to_string(WIND_NOALLOC_MAX_SYSTEMS) + " systems");
-    if (net->builtin == WIND_NET_MASS_ACTION && mass_action_needs_scratch(net, opts.mode))
-        return fail(WIND_ERR_UNSUPPORTED,
-                    "wind_integrate_noalloc: this network's matrices do not fit in LDS (they need an allocation)");
+
     const int stride = opts.const_stride > 0 ? opts.const_stride : net->nconst;
     if (Nsystems == 0 || !(tnow < tend)) return WIND_OK;
     if (!constants || !equations) return fail(WIND_ERR_ARGUMENT, "NULL state or constants");
@@ -796,6 +982,16 @@ extern "C" int wind_integrate_noalloc(const wind_network *net, float tnow, float
     NoAllocDevice &d = g_noalloc[dev];
     rc = noalloc_setup(dev, d);
     if (rc) return rc;
+    {
+        SolverChoice sc0;
+        sc0.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
+        size_t need = 0;
+        rc = scratch_bytes_for(d.num_cu, net, opts.method, opts.mode, opts.kernel, sc0, Nsystems, &need);
+        if (rc) return rc;
+        if (need)
+            return fail(WIND_ERR_UNSUPPORTED,
+                        "wind_integrate_noalloc: this network's matrices do not fit in LDS (they need an allocation)");
+    }
     hipStream_t st = (hipStream_t)opts.stream;  // NULL: the device's null stream
     uint32_t *base = d.scratch;
     unsigned long long *counters = (unsigned long long *)base;
@@ -835,9 +1031,17 @@ extern "C" int wind_integrate_noalloc(const wind_network *net, float tnow, float
         WIND_HIP(hipGetLastError());
         a.index = order;
     }
+    if (opts.trace) {
+        // device pointers only: the trace arrays are the caller's, nothing is staged
+        Pool unused;
+        std::vector<TraceCopy> back;
+        rc = setup_trace(unused, opts, Nsystems, Neqn, &a.trace, &back, st);
+        if (rc) return rc;
+    }
     SolverChoice sc;
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
+    sc.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
     rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(d.num_cu, net, a, opts.mode, opts.kernel, sc, st)
                                  : dispatch<WIND_RK2>(d.num_cu, net, a, opts.mode, opts.kernel, sc, st);
     if (rc) return rc;
@@ -884,8 +1088,9 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     SolverChoice base;
     base.solver = o.solver;
     base.ir_sweeps = o.ir_sweeps;
+    base.pivot = (o.flags & WIND_FLAG_PIVOT) ? 1 : 0;
     v.push_back(base);
-    if (net->builtin != WIND_NET_MASS_ACTION || o.mode != WIND_MODE_FAST) return v;
+    if (net->builtin != WIND_NET_MASS_ACTION || o.mode != WIND_MODE_FAST || base.pivot) return v;
     const int NP = (net->N + 15) & ~15;
     if (NP > 192) return v;
     v.clear();
@@ -899,6 +1104,7 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
             SolverChoice sc;
             sc.solver = sv;
             sc.ir_sweeps = o.ir_sweeps;
+            sc.pivot = base.pivot;
             sc.block = blk;
             v.push_back(sc);
         }
@@ -909,7 +1115,7 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
 
 int tune_bucket(Context &c, const wind_bucket &b, float tnow, float tend, int nfix, float ABS, float REL,
                 const wind_options &o, hipStream_t st, TuneVal *out) {
-    TuneKey key{b.net, o.method, o.mode, o.controller, o.solver, o.ir_sweeps};
+    TuneKey key{b.net, o.method, o.mode, o.controller, o.solver, o.ir_sweeps, (o.flags & WIND_FLAG_PIVOT) ? 1 : 0};
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
         *out = it->second;
@@ -971,6 +1177,9 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
     if (o.solver < WIND_SOLVER_AUTO || o.solver > WIND_SOLVER_MFMA_BF16X3) return fail(WIND_ERR_ARGUMENT, "bad solver");
     if (o.ir_sweeps < 0 || o.ir_sweeps > 4) return fail(WIND_ERR_ARGUMENT, "ir_sweeps must be 0-4");
     if (nfix < 0) return fail(WIND_ERR_ARGUMENT, "n_integration_steps < 0");
+    if (o.trace) return fail(WIND_ERR_ARGUMENT, "trace: supported by wind_integrate only");
+    if ((o.flags & WIND_FLAG_PIVOT) && o.mode != WIND_MODE_FAST)
+        return fail(WIND_ERR_ARGUMENT, "WIND_FLAG_PIVOT applies to fast mode");
     for (int i = 0; i < nb; i++) {
         const wind_bucket &b = buckets[i];
         if (!b.net || b.nsys < 0 || (b.nsys > 0 && (!b.constants || !b.equations)))
@@ -1016,9 +1225,8 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
         int first = -1;
         for (int i : order) {
             size_t b = 0;
-            rc = o.method == WIND_SIE
-                     ? mass_action_scratch_bytes<WIND_SIE>(c.num_cu, buckets[i].net, o.mode, sc[i], buckets[i].nsys, &b)
-                     : mass_action_scratch_bytes<WIND_RK2>(c.num_cu, buckets[i].net, o.mode, sc[i], buckets[i].nsys, &b);
+            rc = scratch_bytes_for(c.num_cu, buckets[i].net, o.method, o.mode, WIND_KERNEL_AUTO, sc[i],
+                                   buckets[i].nsys, &b);
             if (rc) return rc;
             if (b > 0 && first < 0) first = i;
             stream_of[i] = b > 0 ? first : i;
@@ -1077,7 +1285,8 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
             push((uint64_t)(uintptr_t)b.status_out);
             push(((uint64_t)(uint32_t)b.nsys << 32) | (uint32_t)b.const_stride);
             push(((uint64_t)(uint32_t)sc[i].solver << 48) | ((uint64_t)(uint32_t)sc[i].block << 32) |
-                 ((uint64_t)(uint32_t)sc[i].ir_sweeps << 24) | (uint32_t)sc[i].grid);
+                 ((uint64_t)(uint32_t)sc[i].ir_sweeps << 24) | ((uint64_t)(uint32_t)sc[i].pivot << 20) |
+                 (uint32_t)sc[i].grid);
             push((uint64_t)__float_as_uint_host(sc[i].share));
         }
         push((uint64_t)(uintptr_t)cnt);
@@ -1145,7 +1354,10 @@ int wind_launch_async(const wind_network *net, const IntegrateArgs &a, int metho
         int rc = ensure_ready(c);
         if (rc) return rc;
     }
-    if (net->builtin == WIND_NET_MASS_ACTION && mass_action_needs_scratch(net, mode))
+    size_t need = 0;
+    const int rc = scratch_bytes_for(c.num_cu, net, method, mode, kernel, sc, a.nsys, &need);
+    if (rc) return rc;
+    if (need)
         return fail(WIND_ERR_UNSUPPORTED, "this network's matrices do not fit in LDS (it needs the shared scratch buffer)");
     return method == WIND_SIE ? dispatch<WIND_SIE>(c.num_cu, net, a, mode, kernel, sc, st)
                               : dispatch<WIND_RK2>(c.num_cu, net, a, mode, kernel, sc, st);

@@ -19,6 +19,7 @@ namespace wind {
 // ---------------------------------------------------------------------------
 struct MathExact {
     static constexpr bool kFast = false;
+    static constexpr bool kPivot = false;
     __device__ static inline float ex(float x) {
         if (!(x < 88.0f)) x = 88.0f;
         if (x < -87.0f) x = -87.0f;
@@ -62,6 +63,7 @@ struct MathExact {
 
 struct MathFast {
     static constexpr bool kFast = true;
+    static constexpr bool kPivot = false;
     __device__ static inline float ex(float x) { return __expf(x); }
     // log2 on the transcendental unit times ln 2; x must be a normal float
     // (the only caller takes the log of a temperature of at least 10 K)
@@ -69,6 +71,11 @@ struct MathFast {
     __device__ static inline float mad(float a, float b, float c) { return fmaf(a, b, c); }
     // hardware reciprocal (1 ulp) instead of the IEEE division sequence
     __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+};
+
+// Fast arithmetic with partial pivoting in the LU solves (WIND_FLAG_PIVOT).
+struct MathFastPivot : MathFast {
+    static constexpr bool kPivot = true;
 };
 
 // ---------------------------------------------------------------------------
@@ -164,7 +171,9 @@ struct NetHHe6 {
                                         bool want_derivs) {
         s.xe = (y[1] + y[3]) + (y[4] + y[4]);
         s.mu = (((y[0] + y[1]) + (y[2] + y[3])) + y[4]) + s.xe;
-        float dTde = M::div(TUNIT, 1.5f * s.mu);
+        // IEEE division in both modes: the rates' exponents C / T reach ~100,
+        // so a 1-ulp difference in T moves a rate by ~1e-5
+        float dTde = TUNIT / (1.5f * s.mu);
         float Traw = y[5] * dTde;
         bool clamped = !(Traw > TFLOOR);
         float T = clamped ? TFLOOR : Traw;
@@ -173,15 +182,23 @@ struct NetHHe6 {
         s.invT = M::div(1.0f, T);
         s.nHyr = c[0] * YR;
         if constexpr (M::kFast) {
-            // in base 2 on the transcendental unit: A T^b e^(-C/T) =
-            // 2^(log2(e) ln A + b log2 T - log2(e) C / T); T >= 10 K is a normal float
-            constexpr float LOG2E = 1.44269504f;
+            // The exponent x = (ln A + b ln T) - C / T rounds as the exact path
+            // forms it (separate roundings, IEEE 1/T): its size reaches ~100,
+            // so one rounding more or less moves a rate by ~1e-5.  ln T comes
+            // from the hardware log2 (T >= 10 K is a normal float), and e^x is
+            // 2^(x log2 e) with the product split into hi + lo so that the
+            // exponent's rounding does not grow by log2 e.
+            constexpr float LOG2E = 1.44269504f, LOG2E_LO = 1.92596303e-08f, LN2 = 0.693147181f;
             const float l2T = __builtin_amdgcn_logf(T);
-            s.lnT = l2T * 0.693147181f;
+            s.lnT = __fmul_rn(l2T, LN2);
+            const float invT = 1.0f / T;
 #pragma unroll
-            for (int j = 0; j < 7; j++)
-                s.Rj[j] = __builtin_amdgcn_exp2f(fmaf(p.b[j], l2T, p.lnA[j] * LOG2E) - (p.C[j] * LOG2E) * s.invT) *
-                          s.nHyr;
+            for (int j = 0; j < 7; j++) {
+                const float x = __fsub_rn(__fadd_rn(p.lnA[j], __fmul_rn(p.b[j], s.lnT)), __fmul_rn(p.C[j], invT));
+                const float hi = __fmul_rn(x, LOG2E);
+                const float lo = fmaf(x, LOG2E_LO, fmaf(x, LOG2E, -hi));
+                s.Rj[j] = (__builtin_amdgcn_exp2f(hi) * fmaf(lo, LN2, 1.0f)) * s.nHyr;
+            }
             s.sqT = __builtin_amdgcn_sqrtf(T);
         } else {
             s.lnT = M::lg(T);

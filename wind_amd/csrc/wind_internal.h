@@ -52,6 +52,7 @@ struct SolverChoice {
     int block = 0;      // threads per workgroup (dense path); 0 = by N
     int grid = 0;       // workgroups; 0 = fill the device
     float share = 0.0f; // fraction of a full-device grid to use (batches); 0 = all
+    int pivot = 0;      // fast mode: partial pivoting in the LU solves (WIND_FLAG_PIVOT)
 };
 
 // Workgroups for a persistent kernel: a full device (per_cu per CU), scaled

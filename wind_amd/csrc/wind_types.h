@@ -9,7 +9,8 @@ namespace wind {
 // Per-attempt trace of an integration (wind_trace in wind.h, device
 // pointers).  capacity == 0: no trace.  With h set, attempt k of system s
 // takes the step h[s*capacity + k] and its acceptance from accept[...],
-// and the system stops after count[s] attempts (replay); y1, y2 and flag,
+// and the system stops after count[s] attempts (replay), each attempt
+// starting from y_start[(s*capacity + k)*N] when that is set; y1, y2 and flag,
 // when set, receive each attempt's full-step and two-half-step states and
 // the kernel's own error test (1 = its controller would reject).
 struct TraceArgs {
@@ -17,6 +18,7 @@ struct TraceArgs {
     const float *h = nullptr;
     const int32_t *accept = nullptr;
     const int32_t *count = nullptr;
+    const float *y_start = nullptr;  // replay: each attempt's starting state (optional)
     float *y1 = nullptr;
     float *y2 = nullptr;
     int32_t *flag = nullptr;
@@ -39,6 +41,7 @@ struct IntegrateArgs {
     const int32_t *index;          // optional [nsys]: permutation of system ids
     unsigned long long *counters;  // [0] work-queue head, [1] total sub-steps
     TraceArgs trace;               // per-attempt trace / replay (kernels that support it)
+    int32_t pivot = 0;             // fast mode: partial pivoting in the LU solves (launchers pick the kernel)
 };
 
 }  // namespace wind

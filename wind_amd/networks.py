@@ -153,11 +153,53 @@ def mixed_network(N: int):
     return mass_action_network(N, seed=1000 + N)
 
 
+def katz96_spec(name: str = "katz96_gen", ntile: int = 1) -> dict:
+    """The Katz96 H/He network as an expression description for the code
+    generator: the RHS and Jacobian of cuda/ode_system/ode_gold.c:3-93 (the
+    same expressions the reference's python/ode_systems/katz96.py:331-448
+    hands its precompiler), with the electron density ne as a temporary of
+    the first tile and the Jacobian entries in the reference's order,
+    J[13] = J[12] included."""
+    return {
+        "name": name, "species": ["H0", "H+", "He0", "He+", "He++"], "nconst": 10, "ntile": ntile,
+        "temps": [["ne", "y[1]+y[3]+y[4]*2.0"]],
+        "rhs": [
+            "c[2]*ne*y[1] - (c[0]*ne + c[1])*y[0]",
+            "-c[2]*ne*y[1] + (c[0]*ne + c[1])*y[0]",
+            "(c[7]+c[8])*ne*y[3] - (c[3]*ne+c[4])*y[2]",
+            "c[9]*ne*y[4] + (c[3]*ne+c[4])*y[2] - (c[7]+c[8])*ne*y[3] - (c[5]*ne+c[6])*y[3]",
+            "(c[5]*ne+c[6])*y[3] - c[9]*ne*y[4]",
+        ],
+        "jac": [[0, "-(c[0]*ne + c[1])"], [1, "-J[0]"], [6, "-c[2]*ne"], [5, "-J[6]"],
+                [12, "-(c[3]*ne+c[4])"], [13, "J[12]"], [19, "c[5]*ne+c[6]"], [17, "(c[7]+c[8])*ne"],
+                [18, "-J[17] - J[19]"], [24, "-c[9]*ne"], [23, "-J[24]"]],
+    }
+
+
+def nr_test_spec(name: str = "nr_test_gen", ntile: int = 1) -> dict:
+    """NR_test (Numerical Recipes eq. 16.6.1) as the expressions of
+    python/ode_systems/NR_test.py:106-134."""
+    return {
+        "name": name, "species": 2, "nconst": 4, "ntile": ntile,
+        "rhs": ["c[0]*y[0] + c[1]*y[1]", "c[2]*y[0] + c[3]*y[1]"],
+        "jac": [[0, "c[0]"], [1, "c[2]"], [2, "c[1]"], [3, "c[3]"]],
+    }
+
+
+# Katz96 tiled as in the reference's Ntile scaling study (NtileScaling.ipynb,
+# neqn_test.sh): 50 tiles = 250 equations.
+KATZ96_NTILE = 50
+
+
 def generated_specs():
     """Descriptions of the networks that the build generates: the mixed
-    workload's N = 5 and N = 10 networks (as "mixed5", "mixed10"), and a small
+    workload's N = 5 and N = 10 networks (as "mixed5", "mixed10"), a small
     network with one- and two-body reactions and two constants per system
-    ("demo_decay"), and that network tiled three times ("demo_decay_t3")."""
+    ("demo_decay"), that network tiled three times ("demo_decay_t3"), and the
+    reference's networks from their expressions: Katz96 ("katz96_gen"), Katz96
+    tiled 4 and 50 times ("katz96_t4", N = 20; "katz96_t50", N = 250) and
+    NR_test ("nr_test_gen"), and a small system with a zero leading pivot
+    ("pivot_demo", "pivot_demo_t10")."""
     from wind_amd import codegen
 
     specs = []
@@ -176,5 +218,23 @@ def generated_specs():
     }
     specs.append(demo)
     specs.append({**demo, "name": "demo_decay_t3", "ntile": 3})
+    specs.append(katz96_spec())
+    specs.append(katz96_spec("katz96_t4", 4))
+    specs.append(katz96_spec(f"katz96_t{KATZ96_NTILE}", KATZ96_NTILE))
+    specs.append(nr_test_spec())
+    specs.append(pivot_demo_spec())
+    specs.append(pivot_demo_spec("pivot_demo_t10", 10))
     return specs
+
+
+def pivot_demo_spec(name: str = "pivot_demo", ntile: int = 1) -> dict:
+    """A two-species linear system whose Jacobian is [[c0, c1], [c1, -3]]:
+    with c0 = 1 / h, the leading pivot of I - h J is zero (test of the
+    pivoted LU solves).  Tiled 10 times it takes the one-workgroup-per-
+    system kernel (N = 20)."""
+    return {
+        "name": name, "species": 2, "nconst": 2, "ntile": ntile,
+        "rhs": ["c[0]*y[0] + c[1]*y[1]", "c[1]*y[0] - 3.0f*y[1]"],
+        "jac": [[0, "c[0]"], [1, "c[1]"], [2, "c[1]"], [3, "-3.0f"]],
+    }
 
