@@ -761,6 +761,77 @@ void oracle_integrate_traced_batch(int method, const oracle_net *net, float tnow
     for (auto &x : th) x.join();
 }
 
+}  // extern "C"
+
+// Attempt replay (an extension for parity checks; no reference counterpart):
+// the step sequence of a recorded run of integrate_system, on its float
+// time grid, in R arithmetic.
+template <class R>
+void replay_batch(int method, const oracle_net *net, float tnow, const float *constants, int const_stride,
+                  R *equations, int ns, int n, int capacity, const float *h, const int32_t *accept,
+                  const int32_t *count, const float *ystart, R *y1o, R *y2o, int controller, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > ns) nthreads = ns > 0 ? ns : 1;
+    const bool keep_y2 = controller != ORACLE_CTRL_ADAPTIVE && method == ORACLE_SIE;
+    auto run = [&](int lo, int hi) {
+        Work<R> w(n);
+        std::vector<R> c(const_stride > 0 ? const_stride : 1), y(n), y1(n), y2(n);
+        for (int s = lo; s < hi; s++) {
+            for (int k = 0; k < const_stride; k++) c[k] = (R)constants[(size_t)const_stride * s + k];
+            R *ys = equations ? equations + (size_t)n * s : nullptr;
+            if (ys) std::memcpy(y.data(), ys, sizeof(R) * n);
+            float t = tnow;
+            const int cnt = count[s] < capacity ? count[s] : capacity;
+            for (int a = 0; a < cnt; a++) {
+                const size_t at = (size_t)s * capacity + a;
+                if (ystart)
+                    for (int i = 0; i < n; i++) y[i] = (R)ystart[at * n + i];
+                const float tb = t + h[at];
+                // the float run's full step is fl32(tb - t); in double the
+                // interval [t, t + that] reproduces it (and its halves) exactly
+                const R te = std::is_same<R, float>::value ? (R)tb : (R)t + (R)(float)(tb - t);
+                std::memcpy(y1.data(), y.data(), sizeof(R) * n);
+                std::memcpy(y2.data(), y.data(), sizeof(R) * n);
+                take_step<R>(method, net, (R)t, te, 1, y1.data(), c.data(), n, w);
+                take_step<R>(method, net, (R)t, te, 2, y2.data(), c.data(), n, w);
+                if (y1o)
+                    for (int i = 0; i < n; i++) y1o[at * n + i] = y1[i];
+                if (y2o)
+                    for (int i = 0; i < n; i++) y2o[at * n + i] = y2[i];
+                if (accept[at]) {
+                    for (int i = 0; i < n; i++) y[i] = keep_y2 ? y2[i] : 2 * y2[i] - y1[i];
+                    t = tb;
+                }
+            }
+            if (ys && !ystart) std::memcpy(ys, y.data(), sizeof(R) * n);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) {
+        const int lo = (int)((long long)ns * t / nthreads), hi = (int)((long long)ns * (t + 1) / nthreads);
+        th.emplace_back(run, lo, hi);
+    }
+    for (auto &x : th) x.join();
+}
+
+extern "C" {
+
+void oracle_replay_batch(int method, const oracle_net *net, float tnow, const float *constants, int const_stride,
+                         float *equations, int ns, int n, int capacity, const float *h, const int32_t *accept,
+                         const int32_t *count, const float *ystart, float *y1, float *y2, int controller,
+                         int nthreads) {
+    replay_batch<float>(method, net, tnow, constants, const_stride, equations, ns, n, capacity, h, accept, count,
+                        ystart, y1, y2, controller, nthreads);
+}
+
+void oracle_replay_batch_f64(int method, const oracle_net *net, float tnow, const float *constants,
+                             int const_stride, double *equations, int ns, int n, int capacity, const float *h,
+                             const int32_t *accept, const int32_t *count, const float *ystart, double *y1,
+                             double *y2, int controller, int nthreads) {
+    replay_batch<double>(method, net, tnow, constants, const_stride, equations, ns, n, capacity, h, accept, count,
+                         ystart, y1, y2, controller, nthreads);
+}
+
 void oracle_gje_invert(float *matrix, float *inverse, int N) { gje_invert<float>(matrix, inverse, N); }
 
 void oracle_matvec(const float *matrix, float *vector, int N) {

@@ -140,6 +140,30 @@ void oracle_integrate_traced_batch(int method, const oracle_net *net, float tnow
                                    float *y2, int32_t *count, int32_t *steps_out, int32_t *status_out,
                                    long long max_attempts, int controller, int nthreads);
 
+/* Replays recorded attempt sequences (h and accept of every attempt in the
+ * wind_trace layout above, count[s] attempts of system s, one output
+ * interval starting at tnow) on the recorded run's float time grid: attempt
+ * k runs from t_k to t_k' = fl32(t_k + h_k) with the full step t_k' - t_k
+ * and two half steps of (t_k' - t_k) / 2, as integrate_system forms them,
+ * and an accepted attempt moves t to t_k'.  An accepted attempt keeps
+ * 2 y2 - y1 (adaptive controller, and the reference controller's RK2) or y2
+ * (the reference controller's SIE).
+ *   whole run (ystart null): `equations` [Nsystems*N] starts from the
+ *       initial states and ends with the replayed final states;
+ *   per attempt (ystart [Nsystems*capacity*N]): every attempt starts from
+ *       ystart of that attempt, and y1/y2 (optional) receive its states.
+ * The float version replays the float run bit for bit; the double version
+ * gives the same steps in float64 (the float grid's step sizes are exact in
+ * double), the exact-arithmetic value of each attempt or run. */
+void oracle_replay_batch(int method, const oracle_net *net, float tnow, const float *constants, int const_stride,
+                         float *equations, int Nsystems, int N, int capacity, const float *h,
+                         const int32_t *accept, const int32_t *count, const float *ystart, float *y1, float *y2,
+                         int controller, int nthreads);
+void oracle_replay_batch_f64(int method, const oracle_net *net, float tnow, const float *constants,
+                             int const_stride, double *equations, int Nsystems, int N, int capacity,
+                             const float *h, const int32_t *accept, const int32_t *count, const float *ystart,
+                             double *y1, double *y2, int controller, int nthreads);
+
 /* Multi-threaded variant of oracle_gold_integrate for the CPU baseline:
  * systems are split into contiguous slices, one slice per thread. */
 int oracle_gold_integrate_mt(int method, const oracle_net *net,

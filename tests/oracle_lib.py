@@ -47,6 +47,9 @@ def lib():
         L.oracle_integrate_traced_batch.argtypes = [i, vp, f, f, i, vp, i, vp, i, i, f, f, i, vp, vp, vp, vp, vp,
                                                     vp, vp, vp, C.c_longlong, i, i]
         L.oracle_integrate_traced_batch.restype = None
+        for n in ("oracle_replay_batch", "oracle_replay_batch_f64"):
+            getattr(L, n).argtypes = [i, vp, f, vp, i, vp, i, i, i, vp, vp, vp, vp, vp, vp, i, i]
+            getattr(L, n).restype = None
         L.oracle_gje_invert.argtypes = [vp, vp, i]
         L.oracle_gje_invert.restype = None
         L.oracle_matvec.argtypes = [vp, vp, i]
@@ -203,6 +206,53 @@ def traced_batch(method, net, tnow, tend, nfix, constants, equations, nsys, ABS,
                                     count.ctypes.data, steps.ctypes.data, status.ctypes.data, max_attempts,
                                     controller, threads)
     return dict(h=h, accept=acc, flag=flag, y1=y1, y2=y2, count=count, steps=steps, status=status)
+
+
+def start_states(tr, y0, n, N, extrap):
+    """The recorded run's state before each attempt [n, capacity, N]: the
+    initial state, then the value kept at the last accepted attempt
+    (2 y2 - y1 in float32 when extrap, else y2: the reference controller's
+    SIE)."""
+    cap = int(tr["capacity"])
+    acc = tr["accept"].astype(bool) & (np.arange(cap)[None, :] < np.abs(tr["count"])[:, None])
+    kept = (np.float32(2) * tr["y2"] - tr["y1"]).astype(np.float32) if extrap else tr["y2"]
+    idx = np.where(acc, np.arange(cap)[None, :], -1)
+    last = np.maximum.accumulate(idx, axis=1)
+    prev = np.concatenate([np.full((n, 1), -1), last[:, :-1]], axis=1)
+    ys = np.take_along_axis(kept, np.maximum(prev, 0)[:, :, None], axis=1)
+    init = np.broadcast_to(np.asarray(y0, np.float32).reshape(n, 1, N), ys.shape)
+    return np.ascontiguousarray(np.where(prev[:, :, None] >= 0, ys, init), dtype=np.float32)
+
+
+def replay_batch(method, net, tnow, constants, equations, nsys, tr, ystart=None, controller=0, threads=8,
+                 f64=True):
+    """Replays a trace (dict with h, accept, count and capacity, as
+    traced_batch returns) on the recorded run's float time grid, in float64
+    (f64) or float32.  Whole run (ystart None): returns the final states
+    [nsys, N].  Per attempt (ystart [nsys, capacity, N]): returns (y1, y2),
+    each [nsys, capacity, N], every attempt started from its ystart."""
+    L = lib()
+    N = net.N
+    cap = int(tr["capacity"])
+    dt = np.float64 if f64 else np.float32
+    stride = len(constants) // max(nsys, 1)
+    c = _f32(constants)
+    h = _f32(tr["h"])
+    acc = np.ascontiguousarray(tr["accept"], np.int32)
+    cnt = np.ascontiguousarray(np.abs(tr["count"]), np.int32)
+    fn = L.oracle_replay_batch_f64 if f64 else L.oracle_replay_batch
+    if ystart is None:
+        y = np.ascontiguousarray(np.asarray(equations, dt).reshape(nsys, N)).copy()
+        fn(method, net.ref, tnow, c.ctypes.data, stride, y.ctypes.data, nsys, N, cap, h.ctypes.data, acc.ctypes.data,
+           cnt.ctypes.data, None, None, None, controller, threads)
+        return y
+    ys = _f32(ystart)
+    assert ys.shape == (nsys, cap, N)
+    y1 = np.zeros((nsys, cap, N), dt)
+    y2 = np.zeros((nsys, cap, N), dt)
+    fn(method, net.ref, tnow, c.ctypes.data, stride, None, nsys, N, cap, h.ctypes.data, acc.ctypes.data,
+       cnt.ctypes.data, ys.ctypes.data, y1.ctypes.data, y2.ctypes.data, controller, threads)
+    return y1, y2
 
 
 # ----------------------------------------------------------------- reference

@@ -11,26 +11,30 @@ builds print the full-step and two-half-step states of every attempt
 * fast mode (the benchmarked kernels: the lane kernel on hhe6, SIE and RK2,
   and the dense LDS/MFMA solver on mass-action networks at N = 32, 64, 157)
   replays the oracle's float32 run - the same step sizes and the same
-  accept decisions - and is compared with it per species as
-  |y_gpu - y_oracle| / max(|y_oracle|, floor), the floor being the
+  accept decisions - and is compared per species with it and with the same
+  steps in float64 (the oracle's replay in double on the run's float time
+  grid), as |y - y_ref| / max(|y_ref|, floor), the floor being the
   controller's absolute tolerance ABS for hhe6 and 1e-6 of the system's
   total abundance for mass action (the accuracy floor of the benchmark's
   criterion):
   - per attempt: every attempt starts from the oracle's state before it
     (wind_trace.y_start); the full-step and two-half-step states y1, y2 of
-    every accepted attempt are within 1e-5 of the oracle's, relative to the
-    attempt's own magnitudes max(|y_oracle|, |y_start|, |y_oracle - y_start|,
-    floor) (an attempt computes y_start + (its increment), so its rounding
-    scales with both; for explicit RK2 the increment nearly cancels the
-    state in places); the kernel's own error test agrees with the oracle's on
+    every accepted attempt are within 1e-5 of the float oracle's, relative
+    to the attempt's own magnitudes max(|y_oracle|, |y_start|,
+    |y_oracle - y_start|, floor) (an attempt computes y_start + (its
+    increment), so its rounding scales with both; for explicit RK2 the
+    increment nearly cancels the state in places); where the two float
+    results differ by more than 1e-5 (at most 1e-4 of the values: each
+    carries its own float rounding, and the oracle's float result is itself
+    up to ~8e-6 from the float64 attempt), the kernel is within 1e-5 of the
+    float64 attempt; the kernel's own error test agrees with the oracle's on
     at least 99.9% of all attempts;
   - whole runs: the kernel carries its own state through the oracle's
-    accepted-step sequence; every system ends within 5e-5 of the oracle and
-    99.9% of the species within 1e-5 (hhe6's rates k = A T^b exp(-C/T) use
-    the hardware exp2 in fast mode, whose relative difference from the
-    oracle's portable exp reaches ~1e-5 per evaluation near the float
-    exponent's range, and a state integrates such differences over its
-    steps).  Explicit RK2 on this stiff network runs at its stability limit,
+    accepted-step sequence and ends as close to the float64 replay of that
+    sequence as the float oracle's own run does (largest and 99th-percentile
+    difference within 1.5x the oracle's; the float oracle itself ends up to
+    ~1e-4 from the float64 replay on hhe6), and within 5e-5 of the float
+    oracle.  Explicit RK2 on this stiff network runs at its stability limit,
     where replaying steps without the controller's feedback amplifies any
     rounding difference, so RK2 is compared per attempt only.
 """
@@ -94,19 +98,7 @@ def replay(net, method, mode, tend, nfix, c, y0, n, abs_, rel, controller, tr, r
     return y, t
 
 
-def start_states(tr, y0, n, N, extrap):
-    """The oracle's state before each recorded attempt: the initial state,
-    then the value kept at the last accepted attempt (2 y2 - y1 in float32,
-    or y2 for the reference controller's SIE)."""
-    cap = tr["capacity"]
-    acc = tr["accept"].astype(bool) & (np.arange(cap)[None, :] < tr["count"][:, None])
-    kept = (np.float32(2) * tr["y2"] - tr["y1"]).astype(np.float32) if extrap else tr["y2"]
-    idx = np.where(acc, np.arange(cap)[None, :], -1)
-    last = np.maximum.accumulate(idx, axis=1)
-    prev = np.concatenate([np.full((n, 1), -1), last[:, :-1]], axis=1)
-    ys = np.take_along_axis(kept, np.maximum(prev, 0)[:, :, None], axis=1)
-    init = np.broadcast_to(y0.reshape(n, 1, N), ys.shape)
-    return np.ascontiguousarray(np.where(prev[:, :, None] >= 0, ys, init), dtype=np.float32)
+start_states = O.start_states
 
 
 def rel_dev(y, yr, floor):
@@ -160,26 +152,50 @@ def test_exact_trace_and_replay_bit_identical(method, controller):
 
 
 # ------------------------------------------------------------ per attempt
-def check_attempts(tr, t, ystart, floor):
-    """Per-attempt parity of the recorded y1 / y2 (accepted attempts) and
-    agreement of the own error tests (all attempts); returns (worst, agree)."""
+def attempt_parity(method, onet, c, n, tr, t, ystart, floor, label):
+    """Per-attempt parity of the recorded y1 / y2 of every accepted attempt,
+    against the float32 oracle and against the same attempt in float64 (the
+    oracle's replay in double from the same start state on the same time
+    grid): every species is within REL_BOUND of the float oracle, or, where
+    the two float results differ by more (both carry float rounding: the
+    oracle's own float result differs from the float64 attempt by up to
+    ~1e-5), the kernel's result is within REL_BOUND of the float64 attempt;
+    at most 1e-4 of the values take that second branch.  Scale: the
+    attempt's own magnitudes max(|y_oracle|, |y_start|, |y_oracle - y_start|,
+    floor).  The kernel's own error test agrees with the oracle's on at
+    least 99.9% of all attempts.  Returns the statistics."""
     K = tr["capacity"]
     live = np.arange(K)[None, :] < tr["count"][:, None]
     acc = tr["accept"].astype(bool) & live
-    worst = 0.0
-    for k in ("y1", "y2"):
+    ex = O.replay_batch(method, onet, 0.0, c, None, n, tr, ystart=ystart, controller=api.CTRL_ADAPTIVE,
+                        threads=THREADS)
+    st = {"worst": 0.0, "worst_f64": 0.0, "oracle_f64": 0.0, "over": 0, "values": 0}
+    for k, e in zip(("y1", "y2"), ex):
         ref = tr[k][acc].astype(np.float64)
         got = t[k][acc].astype(np.float64)
-        st = ystart[acc].astype(np.float64)
+        y64 = e[acc]
+        ys = ystart[acc].astype(np.float64)
         assert np.all(np.isfinite(ref)) and np.all(np.isfinite(got)), k
-        fl = floor if np.ndim(floor) == 0 else np.broadcast_to(floor[:, None, :], tr[k].shape)[acc]
-        scale = np.maximum.reduce([np.abs(ref), np.abs(st), np.abs(ref - st), np.broadcast_to(fl, ref.shape)])
+        fl = floor if np.ndim(floor) == 0 else np.asarray(floor)[acc]
+        scale = np.maximum.reduce([np.abs(ref), np.abs(ys), np.abs(ref - ys), np.broadcast_to(fl, ref.shape)])
         dev = np.abs(got - ref) / scale
-        worst = max(worst, float(dev.max()))
-        assert dev.max() <= REL_BOUND, (k, np.unravel_index(np.argmax(dev), dev.shape), float(dev.max()))
-    agree = 1.0 - decision_mismatch(t, tr)
-    assert agree >= 0.999
-    return worst, agree
+        dev64 = np.abs(got - y64) / scale
+        over = dev > REL_BOUND
+        st["worst"] = max(st["worst"], float(dev.max()))
+        st["worst_f64"] = max(st["worst_f64"], float(dev64.max()))
+        st["oracle_f64"] = max(st["oracle_f64"], float((np.abs(ref - y64) / scale).max()))
+        st["over"] += int(over.sum())
+        st["values"] += over.size
+        bad = over & (dev64 > REL_BOUND)
+        assert not bad.any(), (label, k, np.argwhere(bad)[:4], dev[bad][:4], dev64[bad][:4])
+    st["agree"] = 1.0 - decision_mismatch(t, tr)
+    print(f"{label}: {int(tr['count'].sum())} attempts of {n} systems ({int(acc.sum())} accepted); vs float oracle "
+          f"max {st['worst']:.3e} ({st['over']} of {st['values']} values above {REL_BOUND:g}); vs float64 attempt "
+          f"max {st['worst_f64']:.3e} (float oracle vs float64: max {st['oracle_f64']:.3e}); own-test decisions "
+          f"agreeing {st['agree']:.6f}", flush=True)
+    assert st["over"] <= 1e-4 * st["values"]
+    assert st["agree"] >= 0.999
+    return st
 
 
 def traced_call(net, method, tend, c, y0, n, abs_, rel, tr, ystart):
@@ -207,10 +223,7 @@ def test_fast_lane_attempts_hhe6(method, tol, cap):
     ystart = start_states(tr, y0, n, 6, True)
     net = api.Network.builtin(api.NET_HHE6, p)
     t = traced_call(net, method, tend, c, y0, n, tol, tol, tr, ystart)
-    worst, agree = check_attempts(tr, t, ystart, tol)
-    print(f"hhe6 fast per-attempt m{method}: {int(tr['count'].sum())} attempts of {n} systems "
-          f"({int(tr['accept'].sum())} accepted); max rel dev {worst:.3e}; own-test decisions agreeing {agree:.6f}",
-          flush=True)
+    attempt_parity(method, O.hhe6(p), c, n, tr, t, ystart, tol, f"hhe6 fast per-attempt {('SIE', 'RK2')[method]}")
 
 
 DENSE_ATTEMPT_CASES = [(32, 48, 400), (64, 32, 240), (157, 8, 150)]
@@ -240,25 +253,35 @@ def test_fast_dense_attempts(N, ns, cap):
     t = traced_call(net, api.SIE, tend, sc, y0, ns, 1e-10, 8e-5, tr, ystart)
     floor = 1e-6 * np.abs(ystart.astype(np.float64)).sum(axis=2)  # [ns, K]
     floor = np.broadcast_to(floor[:, :, None], ystart.shape)
-    K = tr["capacity"]
-    live = np.arange(K)[None, :] < tr["count"][:, None]
-    acc = tr["accept"].astype(bool) & live
-    worst = 0.0
-    for key in ("y1", "y2"):
-        ref = tr[key][acc].astype(np.float64)
-        got = t[key][acc].astype(np.float64)
-        st = ystart[acc].astype(np.float64)
-        scale = np.maximum.reduce([np.abs(ref), np.abs(st), np.abs(ref - st), floor[acc]])
-        dev = np.abs(got - ref) / scale
-        worst = max(worst, float(dev.max()))
-        assert dev.max() <= REL_BOUND, (key, np.unravel_index(np.argmax(dev), dev.shape), float(dev.max()))
-    agree = 1.0 - decision_mismatch(t, tr)
-    assert agree >= 0.999
-    print(f"dense N={N} fast per-attempt: {int(tr['count'].sum())} attempts of {ns} systems; max rel dev {worst:.3e}; "
-          f"own-test decisions agreeing {agree:.6f}", flush=True)
+    attempt_parity(api.SIE, O.mass_action(N, rx, k), sc, ns, tr, t, ystart, floor, f"dense N={N} fast per-attempt")
 
 
 # -------------------------------------------------------------- whole runs
+def whole_run_parity(method, onet, c, y0, n, N, tr, y, floor, label, t):
+    """The kernel's run through the oracle's step sequence against the float
+    oracle's run and against the same sequence replayed in float64 on the
+    same time grid: the kernel is as close to the float64 replay as the
+    float oracle's own run is (max and 99th percentile of the per-species
+    relative difference within 1.5x the oracle's, plus 1e-7), and within
+    5e-5 of the float oracle."""
+    yo = tr["y"].reshape(n, N).astype(np.float64)
+    yg = y.reshape(n, N).astype(np.float64)
+    ex = O.replay_batch(method, onet, 0.0, c, y0, n, tr, controller=api.CTRL_ADAPTIVE, threads=THREADS)
+    fl = np.broadcast_to(floor, yo.shape)
+    dev = np.abs(yg - yo) / np.maximum(np.abs(yo), fl)
+    e_g = np.abs(yg - ex) / np.maximum(np.abs(ex), fl)
+    e_o = np.abs(yo - ex) / np.maximum(np.abs(ex), fl)
+    q = lambda e: (float(e.max()), float(np.quantile(e, 0.99)), float(np.median(e)))
+    print(f"{label}: {n} systems, attempts mean {tr['count'].mean():.0f} max {tr['count'].max()}; vs float oracle "
+          f"max/p99/median {q(dev)[0]:.2e}/{q(dev)[1]:.2e}/{q(dev)[2]:.2e}; vs float64 replay: kernel "
+          f"{q(e_g)[0]:.2e}/{q(e_g)[1]:.2e}/{q(e_g)[2]:.2e}, float oracle {q(e_o)[0]:.2e}/{q(e_o)[1]:.2e}/"
+          f"{q(e_o)[2]:.2e}; own-test decisions differing {decision_mismatch(t, tr):.2e}", flush=True)
+    assert np.all(np.isfinite(yg))
+    assert q(e_g)[0] <= 1.5 * q(e_o)[0] + 1e-7
+    assert q(e_g)[1] <= 1.5 * q(e_o)[1] + 1e-7
+    assert dev.max() <= 5e-5, np.unravel_index(np.argmax(dev), dev.shape)
+
+
 def test_fast_lane_replay_hhe6_sie():
     """The benchmarked fast lane kernel (SIE, tol 2e-6) carries its own state
     through the oracle's accepted-step sequence over 1 Myr."""
@@ -267,15 +290,7 @@ def test_fast_lane_replay_hhe6_sie():
     tr = oracle_trace(api.SIE, O.hhe6(p), tend, 1, c, y0, n, tol, tol, api.CTRL_ADAPTIVE, 1)
     net = api.Network.builtin(api.NET_HHE6, p)
     y, t = replay(net, api.SIE, api.MODE_FAST, tend, 1, c, y0, n, tol, tol, api.CTRL_ADAPTIVE, tr)
-    dev = rel_dev(y, tr["y"], tol)
-    accepted = tr["accept"].sum(axis=1)
-    frac = float(np.mean(dev <= REL_BOUND))
-    print(f"hhe6 fast replay SIE: {n} systems, attempts mean {tr['count'].mean():.0f} max {tr['count'].max()}, "
-          f"accepted steps min {accepted.min()}; rel dev max {dev.max():.3e}, median {np.median(dev):.3e}, "
-          f"within 1e-5: {frac:.5f}; own-test decisions differing {decision_mismatch(t, tr):.2e}", flush=True)
-    assert np.all(np.isfinite(y))
-    assert frac >= 0.999
-    assert dev.max() <= 5e-5, np.unravel_index(np.argmax(dev), dev.shape)
+    whole_run_parity(api.SIE, O.hhe6(p), c, y0, n, 6, tr, y, tol, "hhe6 fast replay SIE", t)
 
 
 DENSE_CASES = [(32, 96, 0), (64, 32, 240), (157, 8, 150)]
@@ -286,27 +301,25 @@ def test_fast_dense_replay(N, ns, cap):
     """The dense LDS/MFMA solver carries its own state through the oracle's
     accepted-step sequence: over the whole interval for N = 32 (the systems
     with at least 100 accepted steps) and over the first 240 / 150 attempts
-    for N = 64 / 157 (at least 100 accepted steps each)."""
+    for N = 64 / 157 (at least 100 accepted steps each); floor 1e-6 of the
+    system's total abundance."""
     rx, k, y0, sc = dense_case(N, ns)
     tend = W.MASS_ACTION_TEND
-    tr = oracle_trace(api.SIE, O.mass_action(N, rx, k), tend, 1, sc, y0, ns, 1e-10, 8e-5, api.CTRL_ADAPTIVE, 1,
-                      max_attempts=cap)
+    onet = O.mass_action(N, rx, k)
+    tr = oracle_trace(api.SIE, onet, tend, 1, sc, y0, ns, 1e-10, 8e-5, api.CTRL_ADAPTIVE, 1, max_attempts=cap)
     accepted = tr["accept"].sum(axis=1)
     keep = accepted >= 100
     assert keep.sum() >= min(ns, 48)
     net = api.Network.mass_action(N, rx, k)
     y, t = replay(net, api.SIE, api.MODE_FAST, tend, 1, sc, y0, ns, 1e-10, 8e-5, api.CTRL_ADAPTIVE, tr)
-    Y = y.reshape(ns, N)[keep]
-    Yr = tr["y"].reshape(ns, N)[keep]
-    floor = 1e-6 * np.abs(Yr.astype(np.float64)).sum(axis=1, keepdims=True)
-    dev = rel_dev(Y, Yr, floor)
-    frac = float(np.mean(dev <= REL_BOUND))
-    print(f"dense N={N} fast replay: {int(keep.sum())} of {ns} systems, attempts mean {tr['count'].mean():.0f}, "
-          f"accepted min {accepted[keep].min()}; rel dev max {dev.max():.3e}, median {np.median(dev):.3e}, within "
-          f"1e-5: {frac:.5f}; own-test decisions differing {decision_mismatch(t, tr):.2e}", flush=True)
-    assert np.all(np.isfinite(y))
-    assert frac >= 0.999
-    assert dev.max() <= 5e-5
+    sub = {key: tr[key][keep] for key in ("h", "accept", "count", "flag")}
+    sub.update(capacity=tr["capacity"], y=tr["y"].reshape(ns, N)[keep].reshape(-1))
+    Yr = sub["y"].reshape(-1, N).astype(np.float64)
+    floor = 1e-6 * np.abs(Yr).sum(axis=1, keepdims=True)
+    nk = int(keep.sum())
+    whole_run_parity(api.SIE, onet, sc.reshape(ns, -1)[keep].reshape(-1), y0.reshape(ns, N)[keep].reshape(-1), nk, N,
+                     sub, y.reshape(ns, N)[keep], floor, f"dense N={N} fast replay ({nk} of {ns} systems)",
+                     {"flag": t["flag"][keep]})
 
 
 def test_trace_rejected_where_unsupported():

@@ -106,3 +106,80 @@ def test_trace_attempts_replay_through_reference_take_step():
                     y = y2 if method == O.SIE else (np.float32(2) * y2 - y1).astype(np.float32)
                     tnow = tb
             assert np.array_equal(y.view(np.uint32), yk[5 * s:5 * s + 5].view(np.uint32))
+
+
+def replay_case(case, method, controller, n):
+    net, y0, c, stride, tend, _, tol = case(n)
+    y = y0.copy()
+    cap = 30000
+    tr = O.traced_batch(method, net, 0.0, tend, 1, c, y, n, tol, tol, capacity=cap, const_stride=stride,
+                        controller=controller, max_attempts=cap, threads=4)
+    assert np.all(tr["count"] > 0), "capacity too small for the case"
+    tr["capacity"] = cap
+    return net, y0, c, tr, y
+
+
+@pytest.mark.parametrize("case", [nr_case, hhe6_case])
+@pytest.mark.parametrize("method", [O.SIE, O.RK2])
+@pytest.mark.parametrize("controller", [0, 1])
+def test_float_replay_reproduces_the_run(case, method, controller):
+    """Replaying a recorded run in float (one output interval) gives its
+    final states and, attempt by attempt from the recorded start states, its
+    y1 and y2 bit for bit: the replay takes the run's steps on its time
+    grid."""
+    n = 24
+    net, y0, c, tr, yend = replay_case(case, method, controller, n)
+    N = net.N
+    yr = O.replay_batch(method, net, 0.0, c, y0, n, tr, controller=controller, threads=4, f64=False)
+    assert np.array_equal(yr.reshape(-1).view(np.uint32), yend.view(np.uint32))
+    ys = O.start_states(tr, y0, n, N, not (controller == 0 and method == O.SIE))
+    y1, y2 = O.replay_batch(method, net, 0.0, c, None, n, tr, ystart=ys, controller=controller, threads=4,
+                            f64=False)
+    live = np.arange(tr["capacity"])[None, :] < tr["count"][:, None]
+    for got, ref in ((y1, tr["y1"]), (y2, tr["y2"])):
+        a, b = got[live], ref[live]
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a[~np.isnan(a)].view(np.uint32), b[~np.isnan(b)].view(np.uint32))
+
+
+@pytest.mark.parametrize("method", [O.SIE, O.RK2])
+def test_f64_replay_closed_form(method):
+    """NR_test is linear (f = J y, J constant): an attempt from y over
+    [t, t'] is y + h (I - h J)^-1 J y (SIE) or y + h J y (RK2) per sub-step.
+    The float64 replay equals that closed form, evaluated in numpy float64
+    with the step sizes of the float run's time grid (full step
+    fl32(t' - t), half steps of half that), per attempt and over the whole
+    run."""
+    n = 16
+    net, y0, c, tr, _ = replay_case(nr_case, method, 1, n)
+    N = 2
+    ys = O.start_states(tr, y0, n, N, True)
+    y1, y2 = O.replay_batch(method, net, 0.0, c, None, n, tr, ystart=ys, controller=1, threads=4)
+    yw = O.replay_batch(method, net, 0.0, c, y0, n, tr, controller=1, threads=4)
+    cc = c.reshape(n, 4).astype(np.float64)
+    f32 = np.float32
+
+    def sub(J, y, h):
+        if method == O.SIE:
+            return y + h * np.linalg.solve(np.eye(2) - h * J, J @ y)
+        return y + h * (J @ y)
+
+    worst = 0.0
+    for s in range(n):
+        J = np.array([[cc[s, 0], cc[s, 1]], [cc[s, 2], cc[s, 3]]])
+        t = f32(0.0)
+        yc = y0.reshape(n, N)[s].astype(np.float64)
+        for k in range(tr["count"][s]):
+            tb = f32(t + tr["h"][s, k])
+            hf = float(f32(tb - t))
+            a1 = sub(J, ys[s, k].astype(np.float64), hf)
+            a2 = sub(J, sub(J, ys[s, k].astype(np.float64), hf / 2), hf / 2)
+            w1 = sub(J, yc, hf)
+            w2 = sub(J, sub(J, yc, hf / 2), hf / 2)
+            sc = max(1.0, float(np.abs(a1).max()), float(np.abs(a2).max()))
+            worst = max(worst, float(np.abs(y1[s, k] - a1).max()) / sc, float(np.abs(y2[s, k] - a2).max()) / sc)
+            if tr["accept"][s, k]:
+                yc = 2 * w2 - w1
+                t = tb
+        assert np.allclose(yw[s], yc, rtol=1e-12, atol=1e-13), s
+    assert worst <= 1e-11, worst  # float64 rounding times the condition of I - hJ (~1e3)

@@ -1,8 +1,10 @@
 """wind_amd: batched stiff-ODE integration (SIE / RK2 step doubling) on MI355X.
 
-Native code lives in wind_amd/lib (built by ``python -m wind_amd.build``);
-``wind_amd.api`` binds the C-ABI of include/wind.h and ``wind_amd.harness``
-mirrors the reference's Python calling convention (python/wind_harness.py,
-python/ode_systems/ode_base.py).
+Native code lives in wind_amd/lib (built by ``python -m wind_amd.build``):
+the core library with the C-ABI of include/wind.h, the per-network drop-in
+libraries that WIND's own Python frontend loads (python/wind_harness.py), and
+the generated network plugins.  ``wind_amd.api`` binds the C-ABI with ctypes,
+``wind_amd.codegen`` compiles network descriptions, and ``wind_amd.networks``
+holds the benchmark workloads.
 """
-__version__ = "0.1.0"
+__version__ = "0.2.0"

@@ -470,7 +470,7 @@ static inline uint32_t __float_as_uint_host(float f) {
 
 extern "C" {
 
-const char *wind_version(void) { return "wind_amd 0.1 (gfx950)"; }
+const char *wind_version(void) { return "wind_amd 0.2 (gfx950)"; }
 
 const char *wind_last_error(void) { return g_last_error.c_str(); }
 

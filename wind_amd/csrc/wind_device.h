@@ -171,9 +171,7 @@ struct NetHHe6 {
                                         bool want_derivs) {
         s.xe = (y[1] + y[3]) + (y[4] + y[4]);
         s.mu = (((y[0] + y[1]) + (y[2] + y[3])) + y[4]) + s.xe;
-        // IEEE division in both modes: the rates' exponents C / T reach ~100,
-        // so a 1-ulp difference in T moves a rate by ~1e-5
-        float dTde = TUNIT / (1.5f * s.mu);
+        float dTde = M::div(TUNIT, 1.5f * s.mu);
         float Traw = y[5] * dTde;
         bool clamped = !(Traw > TFLOOR);
         float T = clamped ? TFLOOR : Traw;
@@ -182,23 +180,18 @@ struct NetHHe6 {
         s.invT = M::div(1.0f, T);
         s.nHyr = c[0] * YR;
         if constexpr (M::kFast) {
-            // The exponent x = (ln A + b ln T) - C / T rounds as the exact path
-            // forms it (separate roundings, IEEE 1/T): its size reaches ~100,
-            // so one rounding more or less moves a rate by ~1e-5.  ln T comes
-            // from the hardware log2 (T >= 10 K is a normal float), and e^x is
-            // 2^(x log2 e) with the product split into hi + lo so that the
-            // exponent's rounding does not grow by log2 e.
-            constexpr float LOG2E = 1.44269504f, LOG2E_LO = 1.92596303e-08f, LN2 = 0.693147181f;
+            // k_j = 2^(log2(A_j) + b_j log2(T) - C_j log2(e) / T) on the
+            // transcendental unit (T >= 10 K is a normal float).  Per attempt
+            // this stays as close to the float64 value of the attempt as the
+            // exact path's float arithmetic does (tested against the oracle's
+            // float64 replay of the same attempts).
+            constexpr float LOG2E = 1.44269504f;
             const float l2T = __builtin_amdgcn_logf(T);
-            s.lnT = __fmul_rn(l2T, LN2);
-            const float invT = 1.0f / T;
+            s.lnT = l2T * 0.693147181f;
 #pragma unroll
-            for (int j = 0; j < 7; j++) {
-                const float x = __fsub_rn(__fadd_rn(p.lnA[j], __fmul_rn(p.b[j], s.lnT)), __fmul_rn(p.C[j], invT));
-                const float hi = __fmul_rn(x, LOG2E);
-                const float lo = fmaf(x, LOG2E_LO, fmaf(x, LOG2E, -hi));
-                s.Rj[j] = (__builtin_amdgcn_exp2f(hi) * fmaf(lo, LN2, 1.0f)) * s.nHyr;
-            }
+            for (int j = 0; j < 7; j++)
+                s.Rj[j] = __builtin_amdgcn_exp2f(fmaf(p.b[j], l2T, p.lnA[j] * LOG2E) - (p.C[j] * LOG2E) * s.invT) *
+                          s.nHyr;
             s.sqT = __builtin_amdgcn_sqrtf(T);
         } else {
             s.lnT = M::lg(T);
