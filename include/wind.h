@@ -213,12 +213,16 @@ typedef struct wind_options {
 enum wind_flags {
     WIND_FLAG_GRAPH = 1,       /* batch: capture the fork/join launch into a hipGraph and replay it */
     WIND_FLAG_NO_AUTOTUNE = 2, /* batch: use default kernel shapes instead of tuning each bucket */
-    WIND_FLAG_PIVOT = 4        /* fast mode: partial pivoting in the LU solves (the row with the
+    WIND_FLAG_PIVOT = 4,       /* fast mode: partial pivoting in the LU solves (the row with the
                                   largest magnitude in the pivot column; a wave-shuffle search in
                                   the one-workgroup-per-system kernels, selects in the
                                   lane-per-system ones).  Mass-action networks then use the LDS
                                   LU instead of the blocked MFMA solver.  Exact mode rejects it:
                                   the reference's Gauss-Jordan is unpivoted. */
+    WIND_FLAG_DENSE_LDS = 8    /* fast mode, mass-action SIE with 97 <= N <= 160: keep the matrix in
+                                  LDS (one system per CU at N = 157) instead of the default
+                                  register-resident solver (two systems per CU); results agree
+                                  to rounding */
 };
 
 void wind_options_init(wind_options *opts);

@@ -83,7 +83,7 @@ def oracle_trace(method, onet, tend, nfix, c, y0, n, abs_, rel, controller, stri
     return tr
 
 
-def replay(net, method, mode, tend, nfix, c, y0, n, abs_, rel, controller, tr, record=False, stride=0):
+def replay(net, method, mode, tend, nfix, c, y0, n, abs_, rel, controller, tr, record=False, stride=0, flags=0):
     cap = tr["capacity"]
     y = y0.copy()
     steps = np.zeros(n, np.int32)
@@ -93,7 +93,7 @@ def replay(net, method, mode, tend, nfix, c, y0, n, abs_, rel, controller, tr, r
         t["y2"] = np.zeros((n, cap, net.N), np.float32)
     t["flag"] = np.full((n, cap), -1, np.int32)
     api.integrate(net, 0.0, tend, nfix, c, y, n, abs_, rel, method=method, mode=mode, controller=controller,
-                  steps_out=steps, trace=t, const_stride=stride)
+                  steps_out=steps, trace=t, const_stride=stride, flags=flags)
     assert np.array_equal(steps, 3 * tr["count"])
     return y, t
 
@@ -198,14 +198,14 @@ def attempt_parity(method, onet, c, n, tr, t, ystart, floor, label):
     return st
 
 
-def traced_call(net, method, tend, c, y0, n, abs_, rel, tr, ystart):
+def traced_call(net, method, tend, c, y0, n, abs_, rel, tr, ystart, flags=0):
     K = tr["capacity"]
     t = {"capacity": K, "h": tr["h"], "accept": tr["accept"], "count": tr["count"], "y_start": ystart,
          "y1": np.zeros((n, K, net.N), np.float32), "y2": np.zeros((n, K, net.N), np.float32),
          "flag": np.full((n, K), -1, np.int32)}
     steps = np.zeros(n, np.int32)
     api.integrate(net, 0.0, tend, 1, c, y0.copy(), n, abs_, rel, method=method, mode=api.MODE_FAST,
-                  controller=api.CTRL_ADAPTIVE, steps_out=steps, trace=t)
+                  controller=api.CTRL_ADAPTIVE, steps_out=steps, trace=t, flags=flags)
     assert np.array_equal(steps, 3 * tr["count"])
     return t
 
@@ -226,7 +226,9 @@ def test_fast_lane_attempts_hhe6(method, tol, cap):
     attempt_parity(method, O.hhe6(p), c, n, tr, t, ystart, tol, f"hhe6 fast per-attempt {('SIE', 'RK2')[method]}")
 
 
-DENSE_ATTEMPT_CASES = [(32, 48, 400), (64, 32, 240), (157, 8, 150)]
+# N = 157 runs on the register-resident solver by default and on the
+# LDS-resident one with FLAG_DENSE_LDS; N = 32, 64 on the LDS-resident one.
+DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (64, 32, 240, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 
 def dense_case(N, ns):
@@ -238,8 +240,8 @@ def dense_case(N, ns):
     return rx, k, y0, sc
 
 
-@pytest.mark.parametrize("N,ns,cap", DENSE_ATTEMPT_CASES)
-def test_fast_dense_attempts(N, ns, cap):
+@pytest.mark.parametrize("N,ns,cap,flags", DENSE_ATTEMPT_CASES)
+def test_fast_dense_attempts(N, ns, cap, flags):
     """The dense LDS/MFMA solver (FP32 MFMA, no refinement) per attempt, on
     the first attempts of the mixed workload's networks (N = 157: the dense
     workload's network) at the benchmark's tolerances (abs 1e-10, rel 8e-5);
@@ -250,10 +252,10 @@ def test_fast_dense_attempts(N, ns, cap):
                       max_attempts=cap, with_states=True)
     ystart = start_states(tr, y0, ns, N, True)
     net = api.Network.mass_action(N, rx, k)
-    t = traced_call(net, api.SIE, tend, sc, y0, ns, 1e-10, 8e-5, tr, ystart)
+    t = traced_call(net, api.SIE, tend, sc, y0, ns, 1e-10, 8e-5, tr, ystart, flags)
     floor = 1e-6 * np.abs(ystart.astype(np.float64)).sum(axis=2)  # [ns, K]
     floor = np.broadcast_to(floor[:, :, None], ystart.shape)
-    attempt_parity(api.SIE, O.mass_action(N, rx, k), sc, ns, tr, t, ystart, floor, f"dense N={N} fast per-attempt")
+    attempt_parity(api.SIE, O.mass_action(N, rx, k), sc, ns, tr, t, ystart, floor, f"dense N={N} flags {flags} fast per-attempt")
 
 
 # -------------------------------------------------------------- whole runs
@@ -293,11 +295,11 @@ def test_fast_lane_replay_hhe6_sie():
     whole_run_parity(api.SIE, O.hhe6(p), c, y0, n, 6, tr, y, tol, "hhe6 fast replay SIE", t)
 
 
-DENSE_CASES = [(32, 96, 0), (64, 32, 240), (157, 8, 150)]
+DENSE_CASES = [(32, 96, 0, 0), (64, 32, 240, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 
-@pytest.mark.parametrize("N,ns,cap", DENSE_CASES)
-def test_fast_dense_replay(N, ns, cap):
+@pytest.mark.parametrize("N,ns,cap,flags", DENSE_CASES)
+def test_fast_dense_replay(N, ns, cap, flags):
     """The dense LDS/MFMA solver carries its own state through the oracle's
     accepted-step sequence: over the whole interval for N = 32 (the systems
     with at least 100 accepted steps) and over the first 240 / 150 attempts
@@ -311,14 +313,15 @@ def test_fast_dense_replay(N, ns, cap):
     keep = accepted >= 100
     assert keep.sum() >= min(ns, 48)
     net = api.Network.mass_action(N, rx, k)
-    y, t = replay(net, api.SIE, api.MODE_FAST, tend, 1, sc, y0, ns, 1e-10, 8e-5, api.CTRL_ADAPTIVE, tr)
+    y, t = replay(net, api.SIE, api.MODE_FAST, tend, 1, sc, y0, ns, 1e-10, 8e-5, api.CTRL_ADAPTIVE, tr,
+                  flags=flags)
     sub = {key: tr[key][keep] for key in ("h", "accept", "count", "flag")}
     sub.update(capacity=tr["capacity"], y=tr["y"].reshape(ns, N)[keep].reshape(-1))
     Yr = sub["y"].reshape(-1, N).astype(np.float64)
     floor = 1e-6 * np.abs(Yr).sum(axis=1, keepdims=True)
     nk = int(keep.sum())
     whole_run_parity(api.SIE, onet, sc.reshape(ns, -1)[keep].reshape(-1), y0.reshape(ns, N)[keep].reshape(-1), nk, N,
-                     sub, y.reshape(ns, N)[keep], floor, f"dense N={N} fast replay ({nk} of {ns} systems)",
+                     sub, y.reshape(ns, N)[keep], floor, f"dense N={N} flags {flags} fast replay ({nk} of {ns} systems)",
                      {"flag": t["flag"][keep]})
 
 

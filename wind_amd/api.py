@@ -20,7 +20,7 @@ MODE_EXACT, MODE_FAST = 0, 1
 KERNEL_AUTO, KERNEL_BLOCK, KERNEL_LANE = 0, 1, 2
 CTRL_REFERENCE, CTRL_ADAPTIVE = 0, 1
 SOLVER_AUTO, SOLVER_VALU, SOLVER_MFMA_F32, SOLVER_MFMA_BF16, SOLVER_MFMA_BF16X3 = 0, 1, 2, 3, 4
-FLAG_GRAPH, FLAG_NO_AUTOTUNE, FLAG_PIVOT = 1, 2, 4
+FLAG_GRAPH, FLAG_NO_AUTOTUNE, FLAG_PIVOT, FLAG_DENSE_LDS = 1, 2, 4, 8
 NET_KATZ96, NET_NR_TEST, NET_HHE6, NET_MASS_ACTION = 0, 1, 2, 3
 STATUS_OK, STATUS_STALLED, STATUS_EXHAUSTED = 0, 1, 2
 

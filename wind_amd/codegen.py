@@ -35,6 +35,16 @@ applied in list order.  That is the operation order of the library's
 table-driven mass-action network, so a two-body network generated here and
 the same table integrate to identical bits in exact mode.
 
+A description may instead give the network as expressions, the way the
+reference's network classes hand their RHS and Jacobian text to its
+precompiler ("temps", "rhs", "jac_temps", "jac"; the format is described
+above _check_expr_network below).  Katz96 and NR_test are generated from
+their reference expressions this way (networks.katz96_spec, nr_test_spec).
+Networks with N <= 16 get the lane-per-system kernels; larger ones (up to
+N = 1023, the reference's fused-kernel limit) a one-workgroup-per-system
+kernel with the matrices in LDS or in device scratch; the generated code
+loops over tiles, so its size does not grow with ntile.
+
     python -m wind_amd.codegen spec.json
 """
 from __future__ import annotations

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels_dense.h"
+#include "kernels_dense_reg.h"
 #include "wind_internal.h"
 
 using namespace wind;
@@ -37,3 +38,50 @@ WIND_DENSE(WIND_RK2, PREC_VALU)
 WIND_DENSE(WIND_RK2, PREC_MFMA_F32)
 WIND_DENSE(WIND_RK2, PREC_MFMA_BF16)
 WIND_DENSE(WIND_RK2, PREC_MFMA_BF16X3)
+
+// ---------------------------------------------------------------------------
+// Register-resident solver (kernels_dense_reg.h).
+
+bool dense_reg_eligible(const MassActionDevice &d, const SolverChoice &sc) {
+    if (sc.dense_kernel == 1 || !d.cb_ptr) return false;
+    if (sc.pivot || sc.ir_sweeps != 0) return false;
+    if (sc.solver != WIND_SOLVER_AUTO && sc.solver != WIND_SOLVER_MFMA_F32) return false;
+    const int NT = (d.N + 15) / 16;
+    if (NT < kDenseRegMinNT || NT > kDenseRegMaxNT) return false;
+    int dev = 0, max_lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+        return false;
+    // three workgroups per CU, or the LDS-resident solver is the better shape
+    return 3 * (dense_reg_lds_bytes(NT, d.R) + 256) <= (size_t)max_lds;
+}
+
+template <int NT>
+int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
+                        hipStream_t st) {
+    auto kfn = a.trace.capacity > 0 ? ma_dense_reg_integrate<NT, true> : ma_dense_reg_integrate<NT>;
+    const size_t lds = dense_reg_lds_bytes(NT, d.R);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, lds);
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys);
+    hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), lds, st, a, d);
+    e = hipGetLastError();
+    if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+    return WIND_OK;
+}
+
+int launch_dense_reg(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
+                     hipStream_t st) {
+    switch ((d.N + 15) / 16) {
+    case 7: return launch_dense_reg_nt<7>(num_cu, d, a, sc, st);
+    case 8: return launch_dense_reg_nt<8>(num_cu, d, a, sc, st);
+    case 9: return launch_dense_reg_nt<9>(num_cu, d, a, sc, st);
+    case 10: return launch_dense_reg_nt<10>(num_cu, d, a, sc, st);
+    default: return wind_fail(WIND_ERR_UNSUPPORTED, "register-resident dense solver: N out of range");
+    }
+}

@@ -1,0 +1,488 @@
+// Register-resident dense SIE solver for mass-action networks with
+// 16 NT - 15 <= N <= 16 NT (fast mode).  One 256-thread workgroup (four
+// waves) owns one system at a time, systems taken from a device queue, and
+// the matrix A = I - hJ lives in the waves' registers instead of LDS: wave
+// (wi, wj) holds the 16x16 tiles (i, j) with i = wi, j = wj (mod 2), each in
+// the MFMA accumulator layout (lane l holds rows 4(l/16) + q, q = 0..3, of
+// column l % 16).  Without the matrix in LDS a workgroup needs under a
+// third of a CU's LDS and (at three waves per SIMD) a third of its
+// registers, so three systems share a CU and each one's serial parts (the
+// pivot chain of a diagonal block, the barriers) overlap with the others'
+// work.
+//
+// Per pass (step-doubling sub-step) of an attempt:
+// * assembly: the per-reaction rate terms and f as in the LDS solver
+//   (kernels_dense.h, dense_eval, same order, so f and every matrix element
+//   round identically); the matrix is scattered into LDS one 16-column
+//   block at a time (one thread per row, the row's (reaction, column) terms
+//   in that block in sweep order), and the owners of that column block's
+//   tiles load them into registers;
+// * block LU without pivoting, A = [I 0; A21 G I][A11 A12; 0 A22 - A21 G A12],
+//   G = A11^-1, per 16-column panel k, with two barriers:
+//     B: W_kj = G_k A_kj for the row panel (G_k from LDS, A_kj in the
+//        owner's registers; W replaces A_kj and goes to LDS), the column
+//        panel A_ik to LDS, and the forward sweep r_i -= A_ik c_k;
+//     C: the trailing update A_ij -= A_ik W_kj on the matrix cores
+//        (v_mfma_f32_16x16x4f32, operands from LDS, accumulator in place);
+//        the owner of the next diagonal tile updates it first, inverts it in
+//        registers (Gauss-Jordan on its transpose, which the accumulator
+//        layout holds as rows), publishes G_{k+1} and forms c_{k+1} =
+//        G_{k+1} r_{k+1};
+// * backward sweep x_k = c_k - sum_{j>k} W_kj x_j by columns, one barrier per
+//   block, the W tiles still in registers;
+// * y += h x.
+// Tile matrix-vector products reduce over the 16 lanes of a row with DPP
+// shifts in a fixed order; every result is deterministic and independent of
+// the grid.  The step control is the LDS solver's.
+#pragma once
+#include "kernels_dense.h"
+
+namespace wind {
+
+// Column blocks assembled per round: as many as keep the round's tiles
+// (NT x CB of 1 KB) within 30 KB of LDS, so that three workgroups fit in a
+// CU's LDS.
+__host__ __device__ constexpr int dense_reg_cb(int NT) { return NT * NT <= 30 ? NT : 30 / NT; }
+
+// LDS layout of the register-resident solver (offsets in floats, 16-byte
+// aligned): per-system vectors, per-reaction rate terms, reactant indices
+// and rate constants, and a work area shared by the assembly stage and the
+// factorisation's panel buffers.  The term lists stay in global memory.
+struct DenseRegOffsets {
+    int y, yw, y1, f, r, w, hda, hdb, k, ab, fptr, work, total;
+};
+
+__host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R) {
+    const int NP = 16 * NT;
+    auto up = [](int x) { return (x + 3) & ~3; };
+    DenseRegOffsets o;
+    int p = 0;
+    o.y = p;
+    p = up(p + NP);
+    o.yw = p;
+    p = up(p + NP);
+    o.y1 = p;
+    p = up(p + NP);
+    o.f = p;
+    p = up(p + NP);
+    o.r = p;
+    p = up(p + NP);
+    o.w = p;
+    p = up(p + R);
+    o.hda = p;
+    p = up(p + R);
+    o.hdb = p;
+    p = up(p + R);
+    o.k = p;
+    p = up(p + R);
+    o.ab = p;
+    p = up(p + 2 * R);
+    o.fptr = p;
+    p = up(p + NP + 1);
+    o.work = p;
+    const int stage = NT * dense_reg_cb(NT) * 256, lu = (2 * NT + 1) * 256;
+    p = up(p + (stage > lu ? stage : lu));
+    o.total = p;
+    return o;
+}
+
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// Sum over the 16 lanes of each DPP row (lanes 16m .. 16m + 15), as a
+// prefix sum: lane 15 of the row ends with the row's total.
+__device__ inline float row_sum16(float v) {
+    v += dpp_f<0x111>(v);  // row_shr:1
+    v += dpp_f<0x112>(v);  // row_shr:2
+    v += dpp_f<0x114>(v);  // row_shr:4
+    v += dpp_f<0x118>(v);  // row_shr:8
+    return v;
+}
+
+// Inverts a 16x16 tile held in the accumulator layout (lane (c, g) holds
+// T[4g + q][c]) in place.  Read as rows, those registers are T^T in the
+// layout diag_block_invert works on (lane (r, g) holds row r, columns
+// 4g .. 4g + 3); Gauss-Jordan on T^T gives (T^T)^-1 = (T^-1)^T, which read
+// back in the accumulator layout is T^-1.  No pivoting: the leading
+// principal minors of T^T are those of T.
+__device__ inline void tile_invert_acc(f32x4 &m, int lane) {
+    // the lane index made opaque, so that the compiler derives the per-step
+    // lane masks here instead of keeping 32 of them live across the kernel
+    asm volatile("" : "+v"(lane));
+    const int r = lane & 15, g = lane >> 4;
+    float a[4] = {m[0], m[1], m[2], m[3]};
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int kg = k >> 2, kc = k & 3;
+        float pv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) pv[c] = row_bcast(a[c], k);
+        const float f = lane_read(a[kc], r + 16 * kg);
+        const float rp = __builtin_amdgcn_rcpf(bcast(a[kc], k + 16 * kg));
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const float prow = pv[c] * rp;
+            const bool pc = 4 * g + c == k;
+            if (r == k)
+                a[c] = pc ? rp : prow;
+            else
+                a[c] = pc ? -f * rp : fmaf(-f, prow, a[c]);
+        }
+    }
+    m = f32x4{a[0], a[1], a[2], a[3]};
+}
+
+// dst[16 i + 4g + q] (q = 0..3, lanes with c = 15 store) = mode: dst - T v,
+// or T v; T in the accumulator layout, v[c] the lane's column entry.
+template <bool SUBTRACT>
+__device__ inline void tile_matvec_store(const f32x4 &m, float v, float *dst, int lane) {
+    const int c = lane & 15, g = lane >> 4;
+    float p[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = row_sum16(m[q] * v);
+    if (c == 15) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) dst[4 * g + q] = SUBTRACT ? dst[4 * g + q] - p[q] : p[q];
+    }
+}
+
+// TRACE: per-attempt trace and replay through a.trace, as in the other
+// kernels.
+template <int NT, bool TRACE = false>
+__global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a, MassActionDevice net) {
+    constexpr int NP = 16 * NT, NL = (NT + 1) / 2, CB = dense_reg_cb(NT);
+    constexpr int NR = (NT + CB - 1) / CB;
+    extern __shared__ float lds[];
+    const int N = net.N, R = net.R, t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wi = wave >> 1, wj = wave & 1;
+    const int lr = lane & 15, g = lane >> 4;
+    const DenseRegOffsets o = dense_reg_offsets(NT, R);
+    float *y = lds + o.y, *yw = lds + o.yw, *y1 = lds + o.y1, *f = lds + o.f, *rv = lds + o.r;
+    float *w = lds + o.w, *hda = lds + o.hda, *hdb = lds + o.hdb, *kk = lds + o.k;
+    int2 *ab = reinterpret_cast<int2 *>(lds + o.ab);
+    int *fptr = reinterpret_cast<int *>(lds + o.fptr);
+    const int *__restrict__ fent = net.f_ent;
+    const int *__restrict__ cbptr = net.cb_ptr;
+    const int *__restrict__ cbent = net.cb_ent;
+    float *work = lds + o.work;
+    float *Gbuf = work, *Cbuf = work + 256, *Wbuf = work + 256 * (NT + 1);
+    // Per-lane bases of this wave's tiles (tile row i = 2 ta + wi, column
+    // j = 2 tb + wj): the accesses below add compile-time offsets.
+    float *const cA = Cbuf + wi * 256 + lr * 16 + 4 * g;  // A-operand reads of Cbuf tile i: + ta * 512
+    float *const cS = Cbuf + wi * 256 + 64 * g + lr;      // stores to Cbuf tile i: + ta * 512 + q * 16
+    float *const wB = Wbuf + wj * 256 + lane * 4;          // Wbuf tile j: + tb * 512
+    float *const rI = rv + 16 * wi;                        // rows of tile row i: + 32 * ta
+    for (int r = t; r < R; r += 256) {
+        ab[r] = make_int2(net.rx[4 * r], net.rx[4 * r + 1]);
+        kk[r] = net.k[r];
+    }
+    for (int i = t; i <= N; i += 256) fptr[i] = net.f_ptr[i];
+    __shared__ int s_flag, s_sid;
+    __shared__ double s_err[4];
+    f32x4 M[NL][NL];
+    unsigned long long steps = 0;
+    unsigned int *head = reinterpret_cast<unsigned int *>(a.counters);
+    __syncthreads();
+
+    // f and A = I - hs J at yv into f and the register tiles (ends with a
+    // barrier).
+    auto assemble = [&](const float *yv, float scale, float hs) __attribute__((always_inline)) {
+        for (int r = t; r < R; r += 256) {
+            const int2 x = ab[r];
+            const float ks = kk[r] * scale;
+            const float ya = yv[x.x], yb = yv[x.y];
+            w[r] = (ks * ya) * yb;
+            hda[r] = hs * (ks * yb);
+            hdb[r] = hs * (ks * ya);
+        }
+        __syncthreads();
+        // f: the row's reaction list in order, its loads issued 8 at a time
+        if (t < NP) {
+            float acc = 0.0f;
+            if (t < N) {
+                const int e1 = fptr[t + 1];
+                for (int e = fptr[t]; e < e1; e += 8) {
+                    int en[8];
+                    float wv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) en[u] = e + u < e1 ? fent[e + u] : 0;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) wv[u] = w[en[u] >> 1];
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (e + u < e1) acc = (en[u] & 1) ? acc - wv[u] : acc + wv[u];
+                }
+            }
+            f[t] = acc;
+        }
+        // The matrix, CB column blocks per round, through a stage buffer in
+        // tile order (tile (i, cb) of the round at (i * ncb + cb - cb0) * 256,
+        // each tile in the accumulator layout), so that an owner loads a tile
+        // with one 16-byte read per lane.  Thread t writes row t: each element
+        // is its terms summed in order in a register (from 1 on the diagonal),
+        // stored once.
+#pragma unroll
+        for (int rd = 0; rd < NR; rd++) {
+            const int cb0 = rd * CB, cb1 = cb0 + CB < NT ? cb0 + CB : NT, ncb = cb1 - cb0;
+            f32x4 *S4 = reinterpret_cast<f32x4 *>(work);
+            for (int q = t; q < NT * ncb * 64; q += 256) S4[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            __syncthreads();
+            if (t < NP) {
+                const int sr = ((t >> 4) * ncb - cb0) * 256 + 64 * ((t & 15) >> 2) + (t & 3);
+                if ((t >> 4) >= cb0 && (t >> 4) < cb1) work[sr + (t >> 4) * 256 + 4 * (t & 15)] = 1.0f;
+                if (t < N) {
+                    const int e0 = cbptr[t * NT + cb0], e1 = cbptr[t * NT + cb1];
+                    float acc = 0.0f;
+                    bool start = true;
+                    for (int e = e0; e < e1; e += 4) {
+                        int en[4];
+                        float v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) en[u] = e + u < e1 ? cbent[e + u] : 0;
+#pragma unroll
+                        for (int u = 0; u < 4; u++) v[u] = (en[u] & 2) ? hdb[en[u] >> 12] : hda[en[u] >> 12];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            if (e + u < e1) {
+                                const int col = (en[u] >> 2) & 511;
+                                acc = start ? (col == t ? 1.0f : 0.0f) : acc;
+                                acc = (en[u] & 1) ? acc + v[u] : acc - v[u];
+                                start = (en[u] >> 11) & 1;
+                                if (start) work[sr + (col >> 4) * 256 + 4 * (col & 15)] = acc;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int cl = 0; cl < CB; cl++) {
+                const int cb = rd * CB + cl;
+                if (cb < NT && wj == (cb & 1)) {
+#pragma unroll
+                    for (int ta = 0; ta < NL; ta++) {
+                        const int i = 2 * ta + wi;
+                        if (i < NT) M[ta][cb >> 1] = S4[wi * ncb * 64 + lane + (2 * ta * ncb + cl) * 64];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    };
+
+    // Diagonal tile (k, k) of its owner: invert, publish G_k, c_k = G_k r_k.
+    auto diag_step = [&](f32x4 &m, int k) __attribute__((always_inline)) {
+        tile_invert_acc(m, lane);
+#pragma unroll
+        for (int q = 0; q < 4; q++) Gbuf[(4 * g + q) * 16 + lr] = m[q];
+        const float v = rv[16 * k + lr];
+        tile_matvec_store<false>(m, v, rv + 16 * k, lane);
+    };
+
+    // Block LU without pivoting with the forward sweep on rv (= f on entry),
+    // per 16-column panel k (two barriers): B) the row panel W_kj = G_k A_kj
+    // (G_k from LDS; W replaces A_kj and goes to LDS), the column panel A_ik
+    // (i > k) to LDS and r_i -= A_ik c_k; C) A_ij -= A_ik W_kj on the matrix
+    // cores, the owner of the next diagonal tile updating it first, inverting
+    // it and forming c_{k+1} = G_{k+1} r_{k+1}.  Leaves c in rv.
+    auto factor = [&]() __attribute__((always_inline)) {
+        if (t < NP) rv[t] = f[t];
+        __syncthreads();
+        if (wave == 0) diag_step(M[0][0], 0);
+        __syncthreads();
+        for (int k = 0; k < NT - 1; k++) {
+            if (wi == (k & 1)) {
+                const f32x4 gr = *reinterpret_cast<const f32x4 *>(Gbuf + lr * 16 + 4 * g);
+#pragma unroll
+                for (int ta = 0; ta < NL; ta++) {
+                    if (2 * ta + wi != k) continue;
+#pragma unroll
+                    for (int tb = 0; tb < NL; tb++) {
+                        const int j = 2 * tb + wj;
+                        if (j <= k || j >= NT) continue;
+                        f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int q = 0; q < 4; q++) z = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[q], M[ta][tb][q], z, 0, 0, 0);
+                        M[ta][tb] = z;
+                        *reinterpret_cast<f32x4 *>(wB + tb * 512) = z;
+                    }
+                }
+            }
+            if (wj == (k & 1)) {
+                const float ck = rv[16 * k + lr];
+#pragma unroll
+                for (int tb = 0; tb < NL; tb++) {
+                    if (2 * tb + wj != k) continue;
+#pragma unroll
+                    for (int ta = 0; ta < NL; ta++) {
+                        const int i = 2 * ta + wi;
+                        if (i <= k || i >= NT) continue;
+                        const f32x4 m = M[ta][tb];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) cS[ta * 512 + q * 16] = -m[q];
+                        tile_matvec_store<true>(m, ck, rI + 32 * ta, lane);
+                    }
+                }
+            }
+            __syncthreads();
+            const int k1 = k + 1;
+            if (wi == (k1 & 1) && wj == (k1 & 1)) {
+#pragma unroll
+                for (int ta = 0; ta < NL; ta++) {
+                    if (2 * ta + wi != k1) continue;
+                    const f32x4 xa = *reinterpret_cast<const f32x4 *>(cA + ta * 512);
+                    const f32x4 wv = *reinterpret_cast<const f32x4 *>(wB + ta * 512);
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        M[ta][ta] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[q], wv[q], M[ta][ta], 0, 0, 0);
+                    diag_step(M[ta][ta], k1);
+                }
+            }
+#pragma unroll
+            for (int ta = 0; ta < NL; ta++) {
+                const int i = 2 * ta + wi;
+                if (i <= k || i >= NT) continue;
+                const f32x4 xa = *reinterpret_cast<const f32x4 *>(cA + ta * 512);
+#pragma unroll
+                for (int tb = 0; tb < NL; tb++) {
+                    const int j = 2 * tb + wj;
+                    if (j <= k || j >= NT || (i == k1 && j == k1)) continue;
+                    const f32x4 wv = *reinterpret_cast<const f32x4 *>(wB + tb * 512);
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        M[ta][tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[q], wv[q], M[ta][tb], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+    };
+
+    // Backward sweep by columns: x_k = c_k, then c_i -= W_ik x_k above.
+    auto back = [&]() __attribute__((always_inline)) {
+        for (int k = NT - 1; k >= 1; k--) {
+            if (wj == (k & 1)) {
+                const float xk = rv[16 * k + lr];
+#pragma unroll
+                for (int tb = 0; tb < NL; tb++) {
+                    if (2 * tb + wj != k) continue;
+#pragma unroll
+                    for (int ta = 0; ta < NL; ta++) {
+                        const int i = 2 * ta + wi;
+                        if (i >= k) continue;
+                        tile_matvec_store<true>(M[ta][tb], xk, rI + 32 * ta, lane);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    };
+
+    for (;;) {
+        if (t == 0) {
+            const unsigned q = atomicAdd(head, 1u);
+            s_sid = q < (unsigned)a.nsys ? (a.index ? a.index[q] : (int)q) : a.nsys;
+        }
+        __syncthreads();
+        const int sid = s_sid;
+        if (sid >= a.nsys) break;
+        for (int i = t; i < N; i += 256) y[i] = a.equations[(size_t)sid * N + i];
+        const float scale = a.constants[(size_t)sid * a.const_stride];
+        __syncthreads();
+        float tnow = a.tnow, h = a.h0;
+        int unsolved = 0, attempts = 0, status = WIND_STATUS_OK;
+        const bool replay = TRACE && a.trace.h != nullptr;
+        const int rcount = replay ? min(a.trace.count[sid], a.trace.capacity) : 0;
+        while (replay ? attempts < rcount : tnow < a.tend) {
+            const size_t slot = TRACE ? (size_t)sid * a.trace.capacity + attempts : 0;
+            if (replay && a.trace.y_start) {
+                for (int i = t; i < N; i += 256) y[i] = a.trace.y_start[slot * N + i];
+                __syncthreads();
+            }
+            const float hh = replay ? a.trace.h[slot] : fminf(a.tend - tnow, h);
+            const float tb = tnow + hh;
+            const float hs_full = tb - tnow, hs_half = (tb - tnow) / 2.0f;
+            for (int pass = 0; pass < 3; pass++) {
+                const float hs = pass == 0 ? hs_full : hs_half;
+                if (pass != 2) {
+                    for (int i = t; i < N; i += 256) yw[i] = y[i];
+                    __syncthreads();
+                }
+                assemble(yw, scale, hs);
+                factor();
+                back();
+                for (int i = t; i < N; i += 256) yw[i] = fmaf(hs, rv[i], yw[i]);
+                __syncthreads();
+                if (pass == 0)
+                    for (int i = t; i < N; i += 256) y1[i] = yw[i];
+            }
+            if (t == 0) s_flag = 0;
+            __syncthreads();
+            bool flag = false;
+            double err = 0.0;
+            if (a.controller == WIND_CTRL_REFERENCE) {
+                bool fail = false;
+                for (int i = t; i < N; i += 256) fail = fail || species_fails<WIND_SIE>(y1[i], yw[i], a.ABS, a.REL);
+                if (fail) s_flag = 1;
+            } else {
+                for (int i = t; i < N; i += 256) err = err_max(err, err_term(y1[i], yw[i], a.ABS, a.REL));
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) err = err_max(err, __shfl_xor(err, off));
+                if (lane == 0) s_err[wave] = err;
+            }
+            __syncthreads();
+            flag = s_flag != 0;
+            if (a.controller != WIND_CTRL_REFERENCE) {
+                err = s_err[0];
+                for (int wv = 1; wv < 4; wv++) err = err_max(err, s_err[wv]);
+            }
+            const bool extrap = a.controller != WIND_CTRL_REFERENCE;
+            if constexpr (TRACE) {
+                if (attempts < a.trace.capacity) {
+                    for (int i = t; i < N; i += 256) {
+                        if (a.trace.y1) a.trace.y1[slot * N + i] = y1[i];
+                        if (a.trace.y2) a.trace.y2[slot * N + i] = yw[i];
+                    }
+                    if (t == 0 && a.trace.flag)
+                        a.trace.flag[slot] = a.controller == WIND_CTRL_REFERENCE ? (int)flag : !(err <= 1.0);
+                }
+                if (replay) {
+                    attempts++;
+                    if (a.trace.accept[slot]) {
+                        tnow = tnow + hh;
+                        for (int i = t; i < N; i += 256) y[i] = extrap ? 2.0f * yw[i] - y1[i] : yw[i];
+                    }
+                    __syncthreads();
+                    continue;
+                }
+            }
+            attempts++;
+            const bool acc = control_step<WIND_SIE, true>(a.controller, a.tend, hh, flag, err, tnow, h, unsolved, status);
+            if (acc)
+                for (int i = t; i < N; i += 256) y[i] = extrap ? 2.0f * yw[i] - y1[i] : yw[i];
+            __syncthreads();
+            if (status != WIND_STATUS_OK) break;
+            if (attempts >= a.max_attempts && tnow < a.tend) {
+                status = WIND_STATUS_EXHAUSTED;
+                break;
+            }
+        }
+        __syncthreads();
+        for (int i = t; i < N; i += 256) a.equations[(size_t)sid * N + i] = y[i];
+        if (t == 0) {
+            if (a.steps_out) a.steps_out[sid] = 3 * attempts;
+            if (a.status_out) a.status_out[sid] = status;
+            steps += 3ull * (unsigned long long)attempts;
+        }
+        __syncthreads();
+    }
+    if (t == 0 && steps) atomicAdd(a.counters + 1, steps);
+}
+
+// LDS bytes of ma_dense_reg_integrate (dynamic part) for a network.
+inline size_t dense_reg_lds_bytes(int NT, int R) { return sizeof(float) * (size_t)dense_reg_offsets(NT, R).total; }
+
+}  // namespace wind

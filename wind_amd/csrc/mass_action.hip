@@ -67,6 +67,37 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
         r_z.insert(r_z.end(), rows[i].begin(), rows[i].end());
         r_ptr[i + 1] = (int32_t)r_z.size();
     }
+    // Jacobian terms by (row, 16-column block) for the register-resident
+    // dense solver: within a row's block, grouped by column (stable, so each
+    // element keeps the order in which dense_eval applies its terms), the
+    // last term of each element flagged.
+    const int NT = (N + 15) / 16;
+    std::vector<int32_t> cb_ptr, cb_ent;
+    if (NT <= 12 && R < (1 << 19)) {
+        struct Term {
+            int col, r, which, neg;
+        };
+        std::vector<std::vector<Term>> cbl((size_t)16 * NT * NT);
+        for (int i = 0; i < N; i++)
+            for (int32_t e : flist[i]) {
+                const int r = e >> 1, neg = e & 1;
+                const int32_t *x = rx + 4 * r;
+                for (int which = 0; which < 2; which++) {
+                    const int col = x[which];
+                    cbl[(size_t)i * NT + col / 16].push_back(Term{col, r, which, neg});
+                }
+            }
+        cb_ptr.assign(cbl.size() + 1, 0);
+        for (size_t q = 0; q < cbl.size(); q++) {
+            auto &v = cbl[q];
+            std::stable_sort(v.begin(), v.end(), [](const Term &p1, const Term &p2) { return p1.col < p2.col; });
+            for (size_t z = 0; z < v.size(); z++) {
+                const int last = z + 1 == v.size() || v[z + 1].col != v[z].col;
+                cb_ent.push_back((v[z].r << 12) | (last << 11) | (v[z].col << 2) | (v[z].which << 1) | v[z].neg);
+            }
+            cb_ptr[q + 1] = (int32_t)cb_ent.size();
+        }
+    }
     MassActionDevice &d = n->ma;
     d.N = N;
     d.R = R;
@@ -80,12 +111,14 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     if (!rc) rc = upload((void **)&d.j_ent, j_ent.data(), sizeof(int32_t) * j_ent.size());
     if (!rc) rc = upload((void **)&d.r_ptr, r_ptr.data(), sizeof(int32_t) * r_ptr.size());
     if (!rc) rc = upload((void **)&d.r_z, r_z.data(), sizeof(int32_t) * r_z.size());
+    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.cb_ptr, cb_ptr.data(), sizeof(int32_t) * cb_ptr.size());
+    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.cb_ent, cb_ent.data(), sizeof(int32_t) * cb_ent.size());
     return rc;
 }
 
 void mass_action_release(wind_network *n) {
     MassActionDevice &d = n->ma;
-    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z};
+    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.cb_ptr, d.cb_ent};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = MassActionDevice();
@@ -212,6 +245,7 @@ int launch_mass_action(int num_cu, const wind_network *net, const IntegrateArgs 
     const bool fast = mode == WIND_MODE_FAST;
     const MassActionPlan plan = mass_action_plan(d, mode, sc.pivot != 0);
     if (plan.dense) {
+        if (METHOD == WIND_SIE && dense_reg_eligible(d, sc)) return launch_dense_reg(num_cu, d, a, sc, st);
         const size_t lds = plan.lds;
         int T = ((N + 15) & ~15) <= 64 ? 64 : 512;
         if (sc.block == 64 || sc.block == 128 || sc.block == 256 || sc.block == 512) T = sc.block;

@@ -694,6 +694,7 @@ int wind_integrate_outputs(const wind_network *net, double tnow, double tend, in
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
     sc.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
+    sc.dense_kernel = (opts.flags & WIND_FLAG_DENSE_LDS) ? 1 : 0;
     double tcur = tnow;
     WIND_HIP(hipEventRecord(c.out_events[0], st));
     for (int k = 1; k <= n_output_steps; k++) {
@@ -794,6 +795,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
     sc.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
+    sc.dense_kernel = (opts.flags & WIND_FLAG_DENSE_LDS) ? 1 : 0;
     size_t scratch_need = 0;
     rc = scratch_bytes_for(c.num_cu, net, opts.method, opts.mode, opts.kernel, sc, Nsystems, &scratch_need);
     if (rc) return rc;
@@ -1042,6 +1044,7 @@ extern "C" int wind_integrate_noalloc(const wind_network *net, float tnow, float
     sc.solver = opts.solver;
     sc.ir_sweeps = opts.ir_sweeps;
     sc.pivot = (opts.flags & WIND_FLAG_PIVOT) ? 1 : 0;
+    sc.dense_kernel = (opts.flags & WIND_FLAG_DENSE_LDS) ? 1 : 0;
     rc = opts.method == WIND_SIE ? dispatch<WIND_SIE>(d.num_cu, net, a, opts.mode, opts.kernel, sc, st)
                                  : dispatch<WIND_RK2>(d.num_cu, net, a, opts.mode, opts.kernel, sc, st);
     if (rc) return rc;
@@ -1089,6 +1092,7 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     base.solver = o.solver;
     base.ir_sweeps = o.ir_sweeps;
     base.pivot = (o.flags & WIND_FLAG_PIVOT) ? 1 : 0;
+    base.dense_kernel = (o.flags & WIND_FLAG_DENSE_LDS) ? 1 : 0;
     v.push_back(base);
     if (net->builtin != WIND_NET_MASS_ACTION || o.mode != WIND_MODE_FAST || base.pivot) return v;
     const int NP = (net->N + 15) & ~15;
@@ -1097,6 +1101,14 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     // the tile products of the dense solver: FP32 MFMA unless the caller names one
     const int sv = o.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : o.solver;
     const int blocks[4] = {64, 128, 256, 512};
+    // the register-resident solver first where it applies (SIE; the LDS
+    // solver's shapes are tried after it)
+    if (o.method == WIND_SIE) {
+        SolverChoice sc = base;
+        sc.solver = sv;
+        sc.dense_kernel = 2;
+        if (base.dense_kernel != 1 && dense_reg_eligible(net->ma, sc)) v.push_back(sc);
+    }
     {
         for (int blk : blocks) {
             if (NP > 64 && blk == 64) continue;
@@ -1106,6 +1118,7 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
             sc.ir_sweeps = o.ir_sweeps;
             sc.pivot = base.pivot;
             sc.block = blk;
+            sc.dense_kernel = 1;
             v.push_back(sc);
         }
     }
@@ -1115,7 +1128,8 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
 
 int tune_bucket(Context &c, const wind_bucket &b, float tnow, float tend, int nfix, float ABS, float REL,
                 const wind_options &o, hipStream_t st, TuneVal *out) {
-    TuneKey key{b.net, o.method, o.mode, o.controller, o.solver, o.ir_sweeps, (o.flags & WIND_FLAG_PIVOT) ? 1 : 0};
+    TuneKey key{b.net, o.method, o.mode, o.controller, o.solver, o.ir_sweeps,
+                ((o.flags & WIND_FLAG_PIVOT) ? 1 : 0) | ((o.flags & WIND_FLAG_DENSE_LDS) ? 2 : 0)};
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
         *out = it->second;
@@ -1286,7 +1300,7 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
             push(((uint64_t)(uint32_t)b.nsys << 32) | (uint32_t)b.const_stride);
             push(((uint64_t)(uint32_t)sc[i].solver << 48) | ((uint64_t)(uint32_t)sc[i].block << 32) |
                  ((uint64_t)(uint32_t)sc[i].ir_sweeps << 24) | ((uint64_t)(uint32_t)sc[i].pivot << 20) |
-                 (uint32_t)sc[i].grid);
+                 ((uint64_t)(uint32_t)sc[i].dense_kernel << 18) | (uint32_t)sc[i].grid);
             push((uint64_t)__float_as_uint_host(sc[i].share));
         }
         push((uint64_t)(uintptr_t)cnt);

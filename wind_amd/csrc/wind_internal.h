@@ -23,6 +23,13 @@ struct MassActionDevice {
     int32_t *j_ent = nullptr;  //        ((r*2 + use_db)*2) + (sign < 0)
     int32_t *r_ptr = nullptr;  // [N+1]  non-zeros by row (for residuals)
     int32_t *r_z = nullptr;    //        indices into j_pos
+    // Register-resident dense solver (N <= 192): the Jacobian terms of row i
+    // in 16-column block cb at cb_ent[cb_ptr[i*NT + cb] .. cb_ptr[i*NT + cb + 1]),
+    // NT = ceil(N / 16), each (r << 12) | (last << 11) | (column << 2) |
+    // (use_db << 1) | (sign < 0), grouped by column in the order dense_eval
+    // applies them, `last` marking an element's final term.
+    int32_t *cb_ptr = nullptr;  // [16*NT*NT + 1]
+    int32_t *cb_ent = nullptr;  // [8R]
 };
 
 }  // namespace wind
@@ -53,6 +60,7 @@ struct SolverChoice {
     int grid = 0;       // workgroups; 0 = fill the device
     float share = 0.0f; // fraction of a full-device grid to use (batches); 0 = all
     int pivot = 0;      // fast mode: partial pivoting in the LU solves (WIND_FLAG_PIVOT)
+    int dense_kernel = 0;  // dense path: 0 = by size, 1 = LDS-resident matrix, 2 = register-resident
 };
 
 // Workgroups for a persistent kernel: a full device (per_cu per CU), scaled
@@ -74,6 +82,14 @@ int launch_mass_action(int num_cu, const wind_network *net, const wind::Integrat
 template <int METHOD, int PREC>
 int launch_dense(int num_cu, const wind::MassActionDevice &d, const wind::IntegrateArgs &a, int ir, size_t lds,
                  int T, const SolverChoice &sc, hipStream_t st);
+
+// Register-resident dense SIE solver (dense.hip): tiles per dimension it is
+// built for, whether it takes this network and choice (fast mode, FP32
+// tiles, no refinement, two workgroups' LDS per CU), and its launch.
+constexpr int kDenseRegMinNT = 7, kDenseRegMaxNT = 10;
+bool dense_reg_eligible(const wind::MassActionDevice &d, const SolverChoice &sc);
+int launch_dense_reg(int num_cu, const wind::MassActionDevice &d, const wind::IntegrateArgs &a, const SolverChoice &sc,
+                     hipStream_t st);
 
 // Fast-mode lane kernels of the compiled-in networks (lane_fast.hip,
 // compiled with fused multiply-add contraction).
