@@ -68,6 +68,12 @@ PEAK_FP32_TFLOPS = 157.3          # published FP32 vector peak (counts packed du
 PEAK_FP32_UNPACKED_TFLOPS = 78.6  # one FMA per lane per cycle: the ceiling without packed FP32
 
 
+def progress(msg):
+    """One line per finished leg on stderr (the JSON line stays alone on
+    stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def host_cores():
     """Threads for the CPU baseline: the CPUs this process may run on
     (sched_getaffinity), capped by the cgroup CPU quota when one is set.
@@ -282,6 +288,7 @@ def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
     api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=True, max_attempts=100000)  # tune + capture
     torch.cuda.synchronize()
     warm_s = time.perf_counter() - t
+    progress(f"mixed batch: first call (autotune + capture) {warm_s:.1f} s")
     times = []
     for _ in range(steps):
         for b in bk:
@@ -291,6 +298,7 @@ def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
         total, _ = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=True, max_attempts=100000)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
+        progress(f"mixed batch: step {len(times)} {times[-1]:.2f} s")
     sec = float(np.median(times))
     out = {"workload": f"mixed-N mass-action, {nsys} systems, N in {list(W.MIXED_SIZES)} with weights "
                        f"{list(W.MIXED_WEIGHTS)}, rate scales logU[0.1, 10], t in [0, {tend:g}], SIE, adaptive "
@@ -333,6 +341,7 @@ def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
             torch.cuda.synchronize()
             rates[str(b["N"])] = n / (time.perf_counter() - t)
             est += b["nsys"] / rates[str(b["N"])]
+            progress(f"mixed batch: plain kernel, N = {b['N']}: {rates[str(b['N'])]:.1f} systems/s")
         out["plain_gpu"] = {"value": nsys / est, "unit": "systems/s",
                             "kernel": "ma_block_integrate: one workgroup per system, equation per thread, "
                                       "Gauss-Jordan inverse (the reference's algorithm), exact arithmetic, same "
@@ -362,15 +371,23 @@ def dense_roofline(torch, api, W, nsys):
     _, kh = api.integrate(net, 0.0, W.MASS_ACTION_TEND, 1, c, yh, nsys, MA_ABS, MA_REL, mode=api.MODE_FAST,
                           device_pointers=True, controller=api.CTRL_ADAPTIVE, want_kernel_ms=True,
                           max_attempts=100000, cost_hint=steps)
+    # the LDS-resident solver on the same systems (WIND_FLAG_DENSE_LDS)
+    yl = torch.from_numpy(y0).to(dev)
+    _, kl = api.integrate(net, 0.0, W.MASS_ACTION_TEND, 1, c, yl, nsys, MA_ABS, MA_REL, mode=api.MODE_FAST,
+                          device_pointers=True, controller=api.CTRL_ADAPTIVE, want_kernel_ms=True,
+                          max_attempts=100000, flags=api.FLAG_DENSE_LDS)
     return {"workload": f"N = 157 mass-action, first {nsys} of {W.DENSE_NSYS} systems, t in [0, 1e9]",
             "sample": f"first {nsys} of the {W.DENSE_NSYS} systems of the N = 157 workload",
-            "kernel": "ma_dense_integrate<SIE, MFMA_F32>", "systems_per_s": nsys / (kms / 1e3),
+            "kernel": "ma_dense_reg_integrate<10> (register-resident matrix, three systems per CU)",
+            "systems_per_s": nsys / (kms / 1e3),
             "kernel_ms": kms, "substeps": int(tot), "flops_per_substep": F,
             "us_per_substep_per_cu": kms * 1e3 / (float(tot) / 256.0),
             "bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_FP32_TFLOPS,
             "with_cost_hint": {"systems_per_s": nsys / (kh / 1e3), "kernel_ms": kh,
-                               "identical_results": bool(torch.equal(yh.view(torch.int32), y.view(torch.int32)))}}
+                               "identical_results": bool(torch.equal(yh.view(torch.int32), y.view(torch.int32)))},
+            "lds_resident_solver": {"kernel": "ma_dense_integrate<SIE, MFMA_F32> (WIND_FLAG_DENSE_LDS)",
+                                    "systems_per_s": nsys / (kl / 1e3), "kernel_ms": kl}}
 
 
 def dense_precision(torch, api, W, nsys):
@@ -482,6 +499,7 @@ def serving(api, W, seconds, load_frac, seed=5):
     loads = {}
     for frac in sorted({0.5, load_frac, 0.9}):
         res, wall = open_loop(frac)
+        progress(f"serving: offered load {frac:g} done ({len(res)} requests)")
         lat = np.array([r[0] for r in res])
         nsys = np.array([r[1] for r in res], np.float64)
         work = np.array([r[2] for r in res], np.float64)
@@ -566,6 +584,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.copy_probe}))
         return
+    progress("start")
     params = W.hhe6_params()
     y0_h, nH_h = W.hhe6_systems(NSYS, seed=SEED)
     nsys = a.nsys
@@ -601,6 +620,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = nsys * a.steps / elapsed
+    progress(f"headline: {a.steps} timed steps, {value:.4g} systems/s")
     kernel_s = float(np.mean(kms_all)) / 1e3
 
     yf = y.cpu().numpy().reshape(-1, N_SPECIES)
@@ -739,15 +759,24 @@ def main():
                            "launch shape), same controller and tolerance as the headline",
                  "speedup": value / pv}
 
+    progress(f"headline and its variants done: {value:.4g} systems/s")
     rk2 = None if a.no_rk2 else rk2_hhe6(torch, api, W, net, y0, cst, nsys, tol)
+    progress("rk2 leg done")
     c1 = None if a.no_c1 else nr_test_c1(W)
+    progress("NR_test row done")
     mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain,
                                                 not a.no_generated)
+    progress("mixed batch done")
     dense = None if a.no_dense else dense_roofline(torch, api, W, a.dense_nsys)
+    progress("dense N = 157 done")
     precision = None if a.no_dense else dense_precision(torch, api, W, 1024)
+    progress("precision forms done")
     serve = None if a.no_serve else serving(api, W, a.serve_seconds, a.serve_load)
+    progress("serving done")
     cpu = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, REF_TOL, 0)
+    progress("CPU baseline (reference algorithm) done")
     cpu_same = None if a.no_cpu else cpu_baseline(params, y0_h, nH_h, a.cpu_seconds, tol, ctrl)
+    progress("CPU baseline (headline algorithm) done")
 
     line = {
         "metric": "ODE systems/sec @ rel err<=1e-3 vs FP64; p50/p99 request latency",

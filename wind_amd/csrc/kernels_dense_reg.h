@@ -166,8 +166,9 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
     int2 *ab = reinterpret_cast<int2 *>(lds + o.ab);
     int *fptr = reinterpret_cast<int *>(lds + o.fptr);
     const int *__restrict__ fent = net.f_ent;
-    const int *__restrict__ cbptr = net.cb_ptr;
     const int *__restrict__ cbent = net.cb_ent;
+    const int *__restrict__ tptr = net.task_ptr;
+    const int *__restrict__ tent = net.task_ent;
     float *work = lds + o.work;
     float *Gbuf = work, *Cbuf = work + 256, *Wbuf = work + 256 * (NT + 1);
     // Per-lane bases of this wave's tiles (tile row i = 2 ta + wi, column
@@ -222,38 +223,44 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
         // The matrix, CB column blocks per round, through a stage buffer in
         // tile order (tile (i, cb) of the round at (i * ncb + cb - cb0) * 256,
         // each tile in the accumulator layout), so that an owner loads a tile
-        // with one 16-byte read per lane.  Thread t writes row t: each element
-        // is its terms summed in order in a register (from 1 on the diagonal),
-        // stored once.
+        // with one 16-byte read per lane.  The stage starts as the identity;
+        // the round's (row, block) term lists are dealt to all 256 threads
+        // (task lists built on the host, longest first to the least-loaded
+        // thread), and each element is its terms summed in order in a register
+        // (from 1 on the diagonal), stored once.
 #pragma unroll
         for (int rd = 0; rd < NR; rd++) {
             const int cb0 = rd * CB, cb1 = cb0 + CB < NT ? cb0 + CB : NT, ncb = cb1 - cb0;
+            const int p0 = tptr[rd * 256 + t], p1 = tptr[rd * 256 + t + 1];
             f32x4 *S4 = reinterpret_cast<f32x4 *>(work);
-            for (int q = t; q < NT * ncb * 64; q += 256) S4[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int q = t; q < NT * ncb * 64; q += 256) {
+                // chunk q: tile (i, cb0 + cl), lane l: rows 16 i + 4 (l >> 4) + c, column 16 (cb0 + cl) + (l & 15)
+                const int tile = q >> 6, l = q & 63, i = tile / ncb, cl = tile - i * ncb;
+                const int dc = 16 * (cb0 + cl) + (l & 15) - 16 * i - 4 * (l >> 4);
+                S4[q] = f32x4{dc == 0 ? 1.0f : 0.0f, dc == 1 ? 1.0f : 0.0f, dc == 2 ? 1.0f : 0.0f, dc == 3 ? 1.0f : 0.0f};
+            }
             __syncthreads();
-            if (t < NP) {
-                const int sr = ((t >> 4) * ncb - cb0) * 256 + 64 * ((t & 15) >> 2) + (t & 3);
-                if ((t >> 4) >= cb0 && (t >> 4) < cb1) work[sr + (t >> 4) * 256 + 4 * (t & 15)] = 1.0f;
-                if (t < N) {
-                    const int e0 = cbptr[t * NT + cb0], e1 = cbptr[t * NT + cb1];
-                    float acc = 0.0f;
-                    bool start = true;
-                    for (int e = e0; e < e1; e += 4) {
-                        int en[4];
-                        float v[4];
+            for (int p = p0; p < p1; p++) {
+                const int task = tent[p];
+                const int e0 = task & 0xffff, e1 = e0 + ((task >> 16) & 127), row = task >> 23;
+                const int sr = ((row >> 4) * ncb - cb0) * 256 + 64 * ((row & 15) >> 2) + (row & 3);
+                float acc = 0.0f;
+                bool start = true;
+                for (int e = e0; e < e1; e += 4) {
+                    int en[4];
+                    float v[4];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) en[u] = e + u < e1 ? cbent[e + u] : 0;
+                    for (int u = 0; u < 4; u++) en[u] = e + u < e1 ? cbent[e + u] : 0;
 #pragma unroll
-                        for (int u = 0; u < 4; u++) v[u] = (en[u] & 2) ? hdb[en[u] >> 12] : hda[en[u] >> 12];
+                    for (int u = 0; u < 4; u++) v[u] = (en[u] & 2) ? hdb[en[u] >> 12] : hda[en[u] >> 12];
 #pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            if (e + u < e1) {
-                                const int col = (en[u] >> 2) & 511;
-                                acc = start ? (col == t ? 1.0f : 0.0f) : acc;
-                                acc = (en[u] & 1) ? acc + v[u] : acc - v[u];
-                                start = (en[u] >> 11) & 1;
-                                if (start) work[sr + (col >> 4) * 256 + 4 * (col & 15)] = acc;
-                            }
+                    for (int u = 0; u < 4; u++) {
+                        if (e + u < e1) {
+                            const int col = (en[u] >> 2) & 511;
+                            acc = start ? (col == row ? 1.0f : 0.0f) : acc;
+                            acc = (en[u] & 1) ? acc + v[u] : acc - v[u];
+                            start = (en[u] >> 11) & 1;
+                            if (start) work[sr + (col >> 4) * 256 + 4 * (col & 15)] = acc;
                         }
                     }
                 }

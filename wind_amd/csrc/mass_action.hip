@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "kernels_dense.h"
+#include "kernels_dense_reg.h"
 #include "kernels_mass_action.h"
 #include "wind_internal.h"
 
@@ -72,7 +73,7 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     // element keeps the order in which dense_eval applies its terms), the
     // last term of each element flagged.
     const int NT = (N + 15) / 16;
-    std::vector<int32_t> cb_ptr, cb_ent;
+    std::vector<int32_t> cb_ptr, cb_ent, task_ptr, task_ent;
     if (NT <= 12 && R < (1 << 19)) {
         struct Term {
             int col, r, which, neg;
@@ -97,6 +98,43 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
             }
             cb_ptr[q + 1] = (int32_t)cb_ent.size();
         }
+        // Per assembly round (dense_reg_cb(NT) column blocks), the non-empty
+        // (row, block) term lists dealt to the workgroup's 256 threads,
+        // longest first to the least-loaded thread; each task word is
+        // e0 | count << 16 | row << 23 (count <= 127: a longer list makes the
+        // network ineligible).
+        const int CB = dense_reg_cb(NT), NRD = (NT + CB - 1) / CB;
+        bool ok = true;
+        for (int rd = 0; rd < NRD && ok; rd++) {
+            std::vector<std::pair<int, int>> tasks;  // (count, word)
+            for (int i = 0; i < N; i++)
+                for (int cb = rd * CB; cb < std::min(NT, rd * CB + CB); cb++) {
+                    const int e0 = cb_ptr[(size_t)i * NT + cb], cnt = cb_ptr[(size_t)i * NT + cb + 1] - e0;
+                    if (cnt == 0) continue;
+                    if (cnt > 127 || e0 >= 65536) ok = false;
+                    tasks.push_back({cnt, e0 | (cnt << 16) | (i << 23)});
+                }
+            std::stable_sort(tasks.begin(), tasks.end(),
+                             [](const std::pair<int, int> &p1, const std::pair<int, int> &p2) { return p1.first > p2.first; });
+            std::vector<std::vector<int32_t>> per(256);
+            std::vector<long long> load(256, 0);
+            for (auto &tk : tasks) {
+                const int th = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                per[th].push_back(tk.second);
+                load[th] += tk.first;
+            }
+            for (int th = 0; th < 256; th++) {
+                task_ptr.push_back((int32_t)task_ent.size());
+                task_ent.insert(task_ent.end(), per[th].begin(), per[th].end());
+            }
+        }
+        task_ptr.push_back((int32_t)task_ent.size());
+        if (!ok || N > 255) {
+            cb_ptr.clear();
+            cb_ent.clear();
+            task_ptr.clear();
+            task_ent.clear();
+        }
     }
     MassActionDevice &d = n->ma;
     d.N = N;
@@ -113,12 +151,14 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     if (!rc) rc = upload((void **)&d.r_z, r_z.data(), sizeof(int32_t) * r_z.size());
     if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.cb_ptr, cb_ptr.data(), sizeof(int32_t) * cb_ptr.size());
     if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.cb_ent, cb_ent.data(), sizeof(int32_t) * cb_ent.size());
+    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.task_ptr, task_ptr.data(), sizeof(int32_t) * task_ptr.size());
+    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.task_ent, task_ent.data(), sizeof(int32_t) * task_ent.size());
     return rc;
 }
 
 void mass_action_release(wind_network *n) {
     MassActionDevice &d = n->ma;
-    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.cb_ptr, d.cb_ent};
+    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.cb_ptr, d.cb_ent, d.task_ptr, d.task_ent};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = MassActionDevice();

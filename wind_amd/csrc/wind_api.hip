@@ -1103,16 +1103,21 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     const int blocks[4] = {64, 128, 256, 512};
     // the register-resident solver first where it applies (SIE; the LDS
     // solver's shapes are tried after it)
+    bool reg = false;
     if (o.method == WIND_SIE) {
         SolverChoice sc = base;
         sc.solver = sv;
         sc.dense_kernel = 2;
-        if (base.dense_kernel != 1 && dense_reg_eligible(net->ma, sc)) v.push_back(sc);
+        if (base.dense_kernel != 1 && dense_reg_eligible(net->ma, sc)) {
+            v.push_back(sc);
+            reg = true;
+        }
     }
     {
         for (int blk : blocks) {
             if (NP > 64 && blk == 64) continue;
             if (NP <= 32 && blk > 128) continue;
+            if (reg && blk != 512) continue;  // the LDS solver's best shape at these sizes
             SolverChoice sc;
             sc.solver = sv;
             sc.ir_sweeps = o.ir_sweeps;
@@ -1136,7 +1141,9 @@ int tune_bucket(Context &c, const wind_bucket &b, float tnow, float tend, int nf
         return WIND_OK;
     }
     const int N = b.net->N;
-    const int ns = std::min(b.nsys, N <= 16 ? 16384 : (N <= 64 ? 4096 : 1024));
+    // samples that fill the device several times over (the register-resident
+    // dense solver runs three systems per CU)
+    const int ns = std::min(b.nsys, N <= 16 ? 16384 : (N <= 64 ? 4096 : 12 * c.num_cu));
     const size_t bytes = (size_t)ns * N * sizeof(float);
     int rc = reserve(c.tune_state, bytes);
     if (!rc) rc = reserve(c.counters, 4 * sizeof(unsigned long long));

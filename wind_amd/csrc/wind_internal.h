@@ -30,6 +30,11 @@ struct MassActionDevice {
     // applies them, `last` marking an element's final term.
     int32_t *cb_ptr = nullptr;  // [16*NT*NT + 1]
     int32_t *cb_ent = nullptr;  // [8R]
+    // and the assembly's work per round: thread t of round rd takes the task
+    // words task_ent[task_ptr[rd*256 + t] .. task_ptr[rd*256 + t + 1]), each
+    // e0 | count << 16 | row << 23 (a (row, block) term list).
+    int32_t *task_ptr = nullptr;
+    int32_t *task_ent = nullptr;
 };
 
 }  // namespace wind
