@@ -96,6 +96,8 @@ def rel_err(y, truth, floor=1e-3):
 
 
 def _acc_stats(e):
+    if len(e) == 0:
+        return None
     return {"sample": int(len(e)), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
             "median_rel_err": float(np.median(e)), "p90_rel_err": float(np.quantile(e, 0.9)),
             "p99_rel_err": float(np.quantile(e, 0.99)), "max_rel_err": float(np.max(e)),
@@ -122,9 +124,9 @@ def accuracy(yf, nsys):
         parts.append(e)
         out[name] = _acc_stats(e)
         if name == "tail":
-            ns = int(T["n_stratified"])
-            out[name]["stratified"] = _acc_stats(e[:ns])
-            out[name]["most_substeps"] = _acc_stats(e[ns:])
+            strat = (np.arange(len(idx)) < int(T["n_stratified"]))[keep]
+            out[name]["stratified"] = _acc_stats(e[strat])
+            out[name]["most_substeps"] = _acc_stats(e[~strat])
     if not parts:
         return None
     res = _acc_stats(np.concatenate(parts))

@@ -1103,21 +1103,22 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     const int blocks[4] = {64, 128, 256, 512};
     // the register-resident solver first where it applies (SIE; the LDS
     // solver's shapes are tried after it)
-    bool reg = false;
+    // Where the register-resident solver applies it is the only candidate:
+    // the LDS solver rounds differently, and a bucket's results must not
+    // depend on the tuning (every shape of one solver gives the same bits).
     if (o.method == WIND_SIE) {
         SolverChoice sc = base;
         sc.solver = sv;
         sc.dense_kernel = 2;
         if (base.dense_kernel != 1 && dense_reg_eligible(net->ma, sc)) {
             v.push_back(sc);
-            reg = true;
+            return v;
         }
     }
     {
         for (int blk : blocks) {
             if (NP > 64 && blk == 64) continue;
             if (NP <= 32 && blk > 128) continue;
-            if (reg && blk != 512) continue;  // the LDS solver's best shape at these sizes
             SolverChoice sc;
             sc.solver = sv;
             sc.ir_sweeps = o.ir_sweeps;
