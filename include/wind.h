@@ -219,9 +219,9 @@ enum wind_flags {
                                   lane-per-system ones).  Mass-action networks then use the LDS
                                   LU instead of the blocked MFMA solver.  Exact mode rejects it:
                                   the reference's Gauss-Jordan is unpivoted. */
-    WIND_FLAG_DENSE_LDS = 8    /* fast mode, mass-action SIE with 97 <= N <= 160: keep the matrix in
+    WIND_FLAG_DENSE_LDS = 8    /* fast mode, mass-action SIE with 49 <= N <= 160: keep the matrix in
                                   LDS (one system per CU at N = 157) instead of the default
-                                  register-resident solver (two systems per CU); results agree
+                                  register-resident solver (three systems per CU); results agree
                                   to rounding */
 };
 

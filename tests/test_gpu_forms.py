@@ -49,12 +49,16 @@ def rel_err(y, yt, floor=1e-6):
 @pytest.mark.parametrize("N", [10, 32, 64, 157])
 def test_dense_fixed_sequence_vs_fp64(N):
     """Huge tolerances accept every attempt: the step sequence is fixed
-    (h0 = tend/4096, growing 4x).  The fast GPU result is compared with the
-    oracle's FP64 run of the same sequence; its largest and its median error
-    over the systems must stay within 4x those of the oracle's float32 run
-    (plus 1e-6): two float32 solvers with different rounding, compared by
-    the size of their errors rather than system by system."""
-    ns = 32
+    (h0 = tend/4096, growing 4x), so the last steps solve I - hJ with h
+    near tend, where the matrix is nearly singular (J conserves elements)
+    and rounding is amplified by its condition number.  The fast GPU result
+    is compared with the oracle's FP64 run of the same sequence; the median,
+    90th percentile and largest of its errors over 256 systems must stay
+    within 4x those of the oracle's float32 run (plus 1e-6): two float32
+    solvers with different rounding, compared by the distribution of their
+    errors rather than system by system (on a few dozen systems the largest
+    error of either is a noisy statistic)."""
+    ns = 256
     rx, k = W.mass_action_network(N, seed=N)
     y0, sc = W.mass_action_systems(ns, N, seed=900 + N, scale_range=(0.5, 2.0))
     onet = O.mass_action(N, rx, k)
@@ -70,11 +74,11 @@ def test_dense_fixed_sequence_vs_fp64(N):
     ref = y64.reshape(ns, N)
     e_gpu = rel_err(y.reshape(ns, N), ref)
     e_cpu = rel_err(y32.reshape(ns, N), ref)
-    print(f"N {N}: gpu max {e_gpu.max():.2e} median {np.median(e_gpu):.2e}; "
-          f"oracle float32 max {e_cpu.max():.2e} median {np.median(e_cpu):.2e}")
+    qs = (0.5, 0.9, 1.0)
+    q_gpu, q_cpu = np.quantile(e_gpu, qs), np.quantile(e_cpu, qs)
+    print(f"N {N}: gpu median/p90/max {q_gpu}; oracle float32 {q_cpu}")
     assert (steps == steps[0]).all()
-    assert e_gpu.max() <= 4.0 * e_cpu.max() + 1e-6
-    assert np.median(e_gpu) <= 4.0 * np.median(e_cpu) + 1e-6
+    assert (q_gpu <= 4.0 * q_cpu + 1e-6).all()
 
 
 def _truth():

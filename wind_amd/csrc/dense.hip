@@ -78,6 +78,9 @@ int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateAr
 int launch_dense_reg(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
     switch ((d.N + 15) / 16) {
+    case 4: return launch_dense_reg_nt<4>(num_cu, d, a, sc, st);
+    case 5: return launch_dense_reg_nt<5>(num_cu, d, a, sc, st);
+    case 6: return launch_dense_reg_nt<6>(num_cu, d, a, sc, st);
     case 7: return launch_dense_reg_nt<7>(num_cu, d, a, sc, st);
     case 8: return launch_dense_reg_nt<8>(num_cu, d, a, sc, st);
     case 9: return launch_dense_reg_nt<9>(num_cu, d, a, sc, st);
