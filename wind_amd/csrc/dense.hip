@@ -43,7 +43,7 @@ WIND_DENSE(WIND_RK2, PREC_MFMA_BF16X3)
 // Register-resident solver (kernels_dense_reg.h).
 
 bool dense_reg_eligible(const MassActionDevice &d, const SolverChoice &sc) {
-    if (sc.dense_kernel == 1 || !d.cb_ptr || !d.task_ptr) return false;
+    if (sc.dense_kernel == 1 || !d.sc_ent) return false;
     if (sc.pivot || sc.ir_sweeps != 0) return false;
     if (sc.solver != WIND_SOLVER_AUTO && sc.solver != WIND_SOLVER_MFMA_F32) return false;
     const int NT = (d.N + 15) / 16;

@@ -13,10 +13,10 @@
 // Per pass (step-doubling sub-step) of an attempt:
 // * assembly: the per-reaction rate terms and f as in the LDS solver
 //   (kernels_dense.h, dense_eval, same order, so f and every matrix element
-//   round identically); the matrix is scattered into LDS one 16-column
-//   block at a time (one thread per row, the row's (reaction, column) terms
-//   in that block in sweep order), and the owners of that column block's
-//   tiles load them into registers;
+//   round identically); the matrix is formed in an LDS stage a few
+//   16-column blocks at a time (the round's elements dealt to all 256
+//   threads on the host, each element its terms in sweep order), and the
+//   owners of those blocks' tiles load them into registers;
 // * block LU without pivoting, A = [I 0; A21 G I][A11 A12; 0 A22 - A21 G A12],
 //   G = A11^-1, per 16-column panel k, with two barriers:
 //     B: W_kj = G_k A_kj for the row panel (G_k from LDS, A_kj in the
@@ -43,6 +43,10 @@ namespace wind {
 // (NT x CB of 1 KB) within 30 KB of LDS, so that three workgroups fit in a
 // CU's LDS.
 __host__ __device__ constexpr int dense_reg_cb(int NT) { return NT * NT <= 30 ? NT : 30 / NT; }
+
+// Assembly stream words a thread loads at a time (streams are padded to a
+// multiple of it).
+constexpr int kDenseRegBatch = 4;
 
 // LDS layout of the register-resident solver (offsets in floats, 16-byte
 // aligned): per-system vectors, per-reaction rate terms, reactant indices
@@ -166,9 +170,8 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
     int2 *ab = reinterpret_cast<int2 *>(lds + o.ab);
     int *fptr = reinterpret_cast<int *>(lds + o.fptr);
     const int *__restrict__ fent = net.f_ent;
-    const int *__restrict__ cbent = net.cb_ent;
-    const int *__restrict__ tptr = net.task_ptr;
-    const int *__restrict__ tent = net.task_ent;
+    const int *__restrict__ scent = net.sc_ent;
+    const int *__restrict__ scoff = net.sc_off;
     float *work = lds + o.work;
     float *Gbuf = work, *Cbuf = work + 256, *Wbuf = work + 256 * (NT + 1);
     // Per-lane bases of this wave's tiles (tile row i = 2 ta + wi, column
@@ -192,6 +195,21 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
     // f and A = I - hs J at yv into f and the register tiles (ends with a
     // barrier).
     auto assemble = [&](const float *yv, float scale, float hs) __attribute__((always_inline)) {
+        constexpr int B = kDenseRegBatch;
+        // the tiles' old values are dead (every tile is stored again below,
+        // under conditions the compiler cannot prove cover all of them)
+#pragma unroll
+        for (int ta = 0; ta < NL; ta++)
+#pragma unroll
+            for (int tb = 0; tb < NL; tb++) M[ta][tb] = __builtin_nondeterministic_value(M[ta][tb]);
+        // the first words of round 0's term stream, loaded while the rate
+        // terms and f are formed
+        int en[B];
+        {
+            const int *sp = scent + (size_t)scoff[0] * 256 + t;
+#pragma unroll
+            for (int u = 0; u < B; u++) en[u] = sp[u * 256];
+        }
         for (int r = t; r < R; r += 256) {
             const int2 x = ab[r];
             const float ks = kk[r] * scale;
@@ -207,15 +225,15 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
             if (t < N) {
                 const int e1 = fptr[t + 1];
                 for (int e = fptr[t]; e < e1; e += 8) {
-                    int en[8];
+                    int fe[8];
                     float wv[8];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) en[u] = e + u < e1 ? fent[e + u] : 0;
+                    for (int u = 0; u < 8; u++) fe[u] = e + u < e1 ? fent[e + u] : 0;
 #pragma unroll
-                    for (int u = 0; u < 8; u++) wv[u] = w[en[u] >> 1];
+                    for (int u = 0; u < 8; u++) wv[u] = w[fe[u] >> 1];
 #pragma unroll
                     for (int u = 0; u < 8; u++)
-                        if (e + u < e1) acc = (en[u] & 1) ? acc - wv[u] : acc + wv[u];
+                        if (e + u < e1) acc = (fe[u] & 1) ? acc - wv[u] : acc + wv[u];
                 }
             }
             f[t] = acc;
@@ -224,15 +242,22 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
         // tile order (tile (i, cb) of the round at (i * ncb + cb - cb0) * 256,
         // each tile in the accumulator layout), so that an owner loads a tile
         // with one 16-byte read per lane.  The stage starts as the identity;
-        // the round's (row, block) term lists are dealt to all 256 threads
-        // (task lists built on the host, longest first to the least-loaded
-        // thread), and each element is its terms summed in order in a register
-        // (from 1 on the diagonal), stored once.
+        // each thread then walks its term stream (the round's elements dealt
+        // to the 256 threads on the host, interleaved so that a batch of
+        // words is one coalesced load per word): an element is its terms
+        // summed in order in a register (from 1 on the diagonal), stored
+        // once at its stage offset.
 #pragma unroll
         for (int rd = 0; rd < NR; rd++) {
             const int cb0 = rd * CB, cb1 = cb0 + CB < NT ? cb0 + CB : NT, ncb = cb1 - cb0;
-            const int p0 = tptr[rd * 256 + t], p1 = tptr[rd * 256 + t + 1];
+            const int s0 = scoff[rd], len = scoff[rd + 1] - s0;
+            const int *sp = scent + (size_t)s0 * 256 + t;
             f32x4 *S4 = reinterpret_cast<f32x4 *>(work);
+            if (rd > 0) {
+                // this round's first words, loaded before the stage fill
+#pragma unroll
+                for (int u = 0; u < B; u++) en[u] = sp[u * 256];
+            }
             for (int q = t; q < NT * ncb * 64; q += 256) {
                 // chunk q: tile (i, cb0 + cl), lane l: rows 16 i + 4 (l >> 4) + c, column 16 (cb0 + cl) + (l & 15)
                 const int tile = q >> 6, l = q & 63, i = tile / ncb, cl = tile - i * ncb;
@@ -240,30 +265,25 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
                 S4[q] = f32x4{dc == 0 ? 1.0f : 0.0f, dc == 1 ? 1.0f : 0.0f, dc == 2 ? 1.0f : 0.0f, dc == 3 ? 1.0f : 0.0f};
             }
             __syncthreads();
-            for (int p = p0; p < p1; p++) {
-                const int task = tent[p];
-                const int e0 = task & 0xffff, e1 = e0 + ((task >> 16) & 127), row = task >> 23;
-                const int sr = ((row >> 4) * ncb - cb0) * 256 + 64 * ((row & 15) >> 2) + (row & 3);
-                float acc = 0.0f;
-                bool start = true;
-                for (int e = e0; e < e1; e += 4) {
-                    int en[4];
-                    float v[4];
+            float acc = 0.0f;
+            bool start = true;
+            for (int j = 0;;) {
+                float v[B];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) en[u] = e + u < e1 ? cbent[e + u] : 0;
+                for (int u = 0; u < B; u++) v[u] = lds[(unsigned)en[u] >> 16];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) v[u] = (en[u] & 2) ? hdb[en[u] >> 12] : hda[en[u] >> 12];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        if (e + u < e1) {
-                            const int col = (en[u] >> 2) & 511;
-                            acc = start ? (col == row ? 1.0f : 0.0f) : acc;
-                            acc = (en[u] & 1) ? acc + v[u] : acc - v[u];
-                            start = (en[u] >> 11) & 1;
-                            if (start) work[sr + (col >> 4) * 256 + 4 * (col & 15)] = acc;
-                        }
-                    }
+                for (int u = 0; u < B; u++) {
+                    // zero words (padding) load and store nothing that matters
+                    const int e = en[u];
+                    acc = start ? ((e & 4) ? 1.0f : 0.0f) : acc;
+                    acc = (e & 1) ? acc + v[u] : acc - v[u];
+                    start = (e & 2) != 0;
+                    if (start) work[(e >> 3) & 8191] = acc;
                 }
+                j += B;
+                if (j >= len) break;
+#pragma unroll
+                for (int u = 0; u < B; u++) en[u] = sp[(j + u) * 256];
             }
             __syncthreads();
 #pragma unroll

@@ -68,72 +68,69 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
         r_z.insert(r_z.end(), rows[i].begin(), rows[i].end());
         r_ptr[i + 1] = (int32_t)r_z.size();
     }
-    // Jacobian terms by (row, 16-column block) for the register-resident
-    // dense solver: within a row's block, grouped by column (stable, so each
-    // element keeps the order in which dense_eval applies its terms), the
-    // last term of each element flagged.
+    // Assembly streams of the register-resident dense solver.  Each matrix
+    // element (row, col) with Jacobian terms is its terms in the order
+    // dense_eval applies them (the row's reactions in sweep order, the
+    // reaction's first reactant before its second).  Per assembly round
+    // (dense_reg_cb(NT) column blocks) the round's elements are dealt to the
+    // workgroup's 256 threads, most terms first to the least-loaded thread;
+    // a thread's stream is its elements' terms, one word each:
+    //   (LDS float index of the term's h dJ value) << 16 | (stage offset of
+    //   the element) << 3 | (element on the diagonal) << 2 | (element's last
+    //   term) << 1 | (reactant row: the term adds)
+    // Streams are padded with zero words (which store nothing) to the
+    // round's longest, rounded up to kDenseRegBatch words, and interleaved:
+    // word j of thread t of round rd at sc_ent[(sc_off[rd] + j) * 256 + t].
     const int NT = (N + 15) / 16;
-    std::vector<int32_t> cb_ptr, cb_ent, task_ptr, task_ent;
-    if (NT <= 12 && R < (1 << 19)) {
-        struct Term {
-            int col, r, which, neg;
-        };
-        std::vector<std::vector<Term>> cbl((size_t)16 * NT * NT);
-        for (int i = 0; i < N; i++)
+    std::vector<int32_t> sc_off, sc_ent;
+    if (NT >= kDenseRegMinNT && NT <= kDenseRegMaxNT) {
+        const DenseRegOffsets lo = dense_reg_offsets(NT, R);
+        const int CB = dense_reg_cb(NT), NRD = (NT + CB - 1) / CB;
+        bool ok = lo.hdb + R < 65536;
+        std::map<std::pair<int, int>, std::vector<uint32_t>> elem;  // (row, col) -> words without last
+        for (int i = 0; i < N && ok; i++)
             for (int32_t e : flist[i]) {
                 const int r = e >> 1, neg = e & 1;
                 const int32_t *x = rx + 4 * r;
                 for (int which = 0; which < 2; which++) {
-                    const int col = x[which];
-                    cbl[(size_t)i * NT + col / 16].push_back(Term{col, r, which, neg});
+                    const uint32_t hd = (uint32_t)((which ? lo.hdb : lo.hda) + r);
+                    elem[{i, x[which]}].push_back((hd << 16) | ((x[which] == i ? 1u : 0u) << 2) | (uint32_t)neg);
                 }
             }
-        cb_ptr.assign(cbl.size() + 1, 0);
-        for (size_t q = 0; q < cbl.size(); q++) {
-            auto &v = cbl[q];
-            std::stable_sort(v.begin(), v.end(), [](const Term &p1, const Term &p2) { return p1.col < p2.col; });
-            for (size_t z = 0; z < v.size(); z++) {
-                const int last = z + 1 == v.size() || v[z + 1].col != v[z].col;
-                cb_ent.push_back((v[z].r << 12) | (last << 11) | (v[z].col << 2) | (v[z].which << 1) | v[z].neg);
-            }
-            cb_ptr[q + 1] = (int32_t)cb_ent.size();
-        }
-        // Per assembly round (dense_reg_cb(NT) column blocks), the non-empty
-        // (row, block) term lists dealt to the workgroup's 256 threads,
-        // longest first to the least-loaded thread; each task word is
-        // e0 | count << 16 | row << 23 (count <= 127: a longer list makes the
-        // network ineligible).
-        const int CB = dense_reg_cb(NT), NRD = (NT + CB - 1) / CB;
-        bool ok = true;
+        sc_off.push_back(0);
         for (int rd = 0; rd < NRD && ok; rd++) {
-            std::vector<std::pair<int, int>> tasks;  // (count, word)
-            for (int i = 0; i < N; i++)
-                for (int cb = rd * CB; cb < std::min(NT, rd * CB + CB); cb++) {
-                    const int e0 = cb_ptr[(size_t)i * NT + cb], cnt = cb_ptr[(size_t)i * NT + cb + 1] - e0;
-                    if (cnt == 0) continue;
-                    if (cnt > 127 || e0 >= 65536) ok = false;
-                    tasks.push_back({cnt, e0 | (cnt << 16) | (i << 23)});
-                }
-            std::stable_sort(tasks.begin(), tasks.end(),
-                             [](const std::pair<int, int> &p1, const std::pair<int, int> &p2) { return p1.first > p2.first; });
-            std::vector<std::vector<int32_t>> per(256);
-            std::vector<long long> load(256, 0);
-            for (auto &tk : tasks) {
-                const int th = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-                per[th].push_back(tk.second);
-                load[th] += tk.first;
+            const int cb0 = rd * CB, ncb = std::min(NT, cb0 + CB) - cb0;
+            std::vector<std::pair<int, std::vector<uint32_t>>> items;  // (count, words)
+            for (auto &kv : elem) {
+                const int row = kv.first.first, col = kv.first.second;
+                if (col / 16 < cb0 || col / 16 >= cb0 + ncb) continue;
+                const uint32_t dest = (uint32_t)(((row >> 4) * ncb + (col >> 4) - cb0) * 256 + 64 * ((row & 15) >> 2) +
+                                                 (row & 3) + 4 * (col & 15));
+                std::vector<uint32_t> w = kv.second;
+                for (auto &x : w) x |= dest << 3;
+                w.back() |= 2u;
+                items.push_back({(int)w.size(), std::move(w)});
             }
-            for (int th = 0; th < 256; th++) {
-                task_ptr.push_back((int32_t)task_ent.size());
-                task_ent.insert(task_ent.end(), per[th].begin(), per[th].end());
+            std::stable_sort(items.begin(), items.end(),
+                             [](const auto &p1, const auto &p2) { return p1.first > p2.first; });
+            std::vector<std::vector<uint32_t>> per(256);
+            for (auto &it : items) {
+                size_t th = 0;
+                for (size_t u = 1; u < per.size(); u++)
+                    if (per[u].size() < per[th].size()) th = u;
+                per[th].insert(per[th].end(), it.second.begin(), it.second.end());
             }
+            size_t len = kDenseRegBatch;
+            for (auto &p : per) len = std::max(len, (p.size() + kDenseRegBatch - 1) / kDenseRegBatch * kDenseRegBatch);
+            const size_t base = sc_ent.size();
+            sc_ent.resize(base + len * 256, 0);
+            for (int th = 0; th < 256; th++)
+                for (size_t j = 0; j < per[th].size(); j++) sc_ent[base + j * 256 + th] = (int32_t)per[th][j];
+            sc_off.push_back(sc_off.back() + (int32_t)len);
         }
-        task_ptr.push_back((int32_t)task_ent.size());
-        if (!ok || N > 255) {
-            cb_ptr.clear();
-            cb_ent.clear();
-            task_ptr.clear();
-            task_ent.clear();
+        if (!ok) {
+            sc_off.clear();
+            sc_ent.clear();
         }
     }
     MassActionDevice &d = n->ma;
@@ -149,16 +146,14 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     if (!rc) rc = upload((void **)&d.j_ent, j_ent.data(), sizeof(int32_t) * j_ent.size());
     if (!rc) rc = upload((void **)&d.r_ptr, r_ptr.data(), sizeof(int32_t) * r_ptr.size());
     if (!rc) rc = upload((void **)&d.r_z, r_z.data(), sizeof(int32_t) * r_z.size());
-    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.cb_ptr, cb_ptr.data(), sizeof(int32_t) * cb_ptr.size());
-    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.cb_ent, cb_ent.data(), sizeof(int32_t) * cb_ent.size());
-    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.task_ptr, task_ptr.data(), sizeof(int32_t) * task_ptr.size());
-    if (!rc && !cb_ptr.empty()) rc = upload((void **)&d.task_ent, task_ent.data(), sizeof(int32_t) * task_ent.size());
+    if (!rc && !sc_off.empty()) rc = upload((void **)&d.sc_off, sc_off.data(), sizeof(int32_t) * sc_off.size());
+    if (!rc && !sc_off.empty()) rc = upload((void **)&d.sc_ent, sc_ent.data(), sizeof(int32_t) * sc_ent.size());
     return rc;
 }
 
 void mass_action_release(wind_network *n) {
     MassActionDevice &d = n->ma;
-    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.cb_ptr, d.cb_ent, d.task_ptr, d.task_ent};
+    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.sc_off, d.sc_ent};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = MassActionDevice();

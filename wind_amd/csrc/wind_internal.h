@@ -23,18 +23,11 @@ struct MassActionDevice {
     int32_t *j_ent = nullptr;  //        ((r*2 + use_db)*2) + (sign < 0)
     int32_t *r_ptr = nullptr;  // [N+1]  non-zeros by row (for residuals)
     int32_t *r_z = nullptr;    //        indices into j_pos
-    // Register-resident dense solver (N <= 192): the Jacobian terms of row i
-    // in 16-column block cb at cb_ent[cb_ptr[i*NT + cb] .. cb_ptr[i*NT + cb + 1]),
-    // NT = ceil(N / 16), each (r << 12) | (last << 11) | (column << 2) |
-    // (use_db << 1) | (sign < 0), grouped by column in the order dense_eval
-    // applies them, `last` marking an element's final term.
-    int32_t *cb_ptr = nullptr;  // [16*NT*NT + 1]
-    int32_t *cb_ent = nullptr;  // [8R]
-    // and the assembly's work per round: thread t of round rd takes the task
-    // words task_ent[task_ptr[rd*256 + t] .. task_ptr[rd*256 + t + 1]), each
-    // e0 | count << 16 | row << 23 (a (row, block) term list).
-    int32_t *task_ptr = nullptr;
-    int32_t *task_ent = nullptr;
+    // Register-resident dense solver: the assembly rounds' per-thread term
+    // streams (built in mass_action_build, which documents the word format),
+    // round rd's word j of thread t at sc_ent[(sc_off[rd] + j) * 256 + t].
+    int32_t *sc_off = nullptr;  // [rounds + 1]
+    int32_t *sc_ent = nullptr;
 };
 
 }  // namespace wind
