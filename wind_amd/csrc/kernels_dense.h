@@ -88,8 +88,19 @@ __device__ inline f32x4 tile_mma(FX X, FY Y, f32x4 c, int lane) {
     return c;
 }
 
-__device__ inline float lane_read(float v, int src_lane) {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+// Row kg (lanes 16 kg .. 16 kg + 15) of v, copied to all four 16-lane rows
+// with gfx950's lane swaps: the 16-lane swap exchanges row 1 of its first
+// operand with row 0 of its second, and row 3 with row 2; the 32-lane swap
+// exchanges the upper half of the first with the lower half of the second.
+// With both operands v: p[0] holds rows (0, 0, 2, 2), p[1] rows (1, 1, 3, 3),
+// and the 32-lane swap of (a, b, c, d) gives (a, b, a, b) and (c, d, c, d).
+// kg is a compile-time value once the callers' loops are unrolled.
+__device__ inline float row_to_all(float v, int kg) {
+    const unsigned x = __float_as_uint(v);
+    const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    const unsigned y = (kg & 1) ? p[1] : p[0];
+    const auto q = __builtin_amdgcn_permlane32_swap(y, y, false, false);
+    return __uint_as_float((kg & 2) ? q[1] : q[0]);
 }
 
 // Lane n of each 16-lane row, broadcast to the whole row (DPP row_newbcast;
@@ -114,7 +125,7 @@ __device__ inline float row_bcast(float v, int k) {
 // Gauss-Jordan without pivoting).  Executed by one whole wave: lane l holds
 // row l % 16, columns 4(l / 16) .. 4(l / 16) + 3.  The pivot row lies in
 // each lane's own 16-lane row (DPP row broadcast); the pivot column comes
-// from another row of lanes (ds_bpermute).
+// from another row of lanes (row_to_all).
 __device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
     const int r = lane & 15, g = lane >> 4;
     float a[4];
@@ -126,7 +137,7 @@ __device__ inline void diag_block_invert(float *A, int S, int k0, int lane) {
         float pv[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) pv[c] = row_bcast(a[c], k);
-        const float f = lane_read(a[kc], r + 16 * kg);
+        const float f = row_to_all(a[kc], kg);
         const float rp = __builtin_amdgcn_rcpf(bcast(a[kc], k + 16 * kg));
 #pragma unroll
         for (int c = 0; c < 4; c++) {

@@ -123,7 +123,7 @@ __device__ inline void tile_invert_acc(f32x4 &m, int lane) {
         float pv[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) pv[c] = row_bcast(a[c], k);
-        const float f = lane_read(a[kc], r + 16 * kg);
+        const float f = row_to_all(a[kc], kg);
         const float rp = __builtin_amdgcn_rcpf(bcast(a[kc], k + 16 * kg));
 #pragma unroll
         for (int c = 0; c < 4; c++) {
