@@ -302,11 +302,30 @@ def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
         times.append(time.perf_counter() - t)
         progress(f"mixed batch: step {len(times)} {times[-1]:.2f} s")
     sec = float(np.median(times))
+    # The same integration with each bucket started from cost hints (its
+    # sub-step counts from the step before); results are bit-identical.
+    ref_y = [b["equations"].clone() for b in bk]
+    for b in bk:
+        b["cost_hint"] = b["steps_out"].clone()
+        b["equations"].copy_(b["y0"])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=True, max_attempts=100000)
+    torch.cuda.synchronize()
+    hinted_s = time.perf_counter() - t
+    same = all(bool(torch.equal(b["equations"], r)) for b, r in zip(bk, ref_y))
+    for b in bk:
+        del b["cost_hint"]
+    progress(f"mixed batch: step with cost hints {hinted_s:.2f} s (results identical: {same})")
     out = {"workload": f"mixed-N mass-action, {nsys} systems, N in {list(W.MIXED_SIZES)} with weights "
                        f"{list(W.MIXED_WEIGHTS)}, rate scales logU[0.1, 10], t in [0, {tend:g}], SIE, adaptive "
                        f"controller abs {MA_ABS:g} rel {MA_REL:g}",
            "value": nsys / sec, "unit": "systems/s", "ms_per_step": sec * 1e3, "steps": steps,
-           "first_call_s_incl_autotune_and_capture": warm_s, "substeps": int(total), "buckets": {}}
+           "first_call_s_incl_autotune_and_capture": warm_s, "substeps": int(total), "buckets": {},
+           "with_cost_hint": {"value": nsys / hinted_s, "ms_per_step": hinted_s * 1e3,
+                              "results_identical": same,
+                              "hints": "each bucket's steps_out from the step before (one step, graph "
+                                       "captured with the hints in the same call)"}}
     tp = os.path.join(ROOT, "tests", "golden", "truth_mass_action.npz")
     T = np.load(tp) if os.path.exists(tp) else None
     fracs = []

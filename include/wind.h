@@ -285,6 +285,12 @@ typedef struct wind_bucket {
     int const_stride;         /* 0 = network default */
     int32_t *steps_out;       /* optional device [nsys] */
     int32_t *status_out;      /* optional device [nsys] */
+    const int32_t *cost_hint; /* optional device [nsys]: per-system cost
+                                 estimates, as wind_options.cost_hint for
+                                 wind_integrate (e.g. the previous batch's
+                                 steps_out); buckets of kernels with a work
+                                 queue start their systems in decreasing
+                                 order of it.  Results do not depend on it. */
 } wind_bucket;
 
 /* Integrates every bucket from tnow to tend in one fork/join launch: one

@@ -198,6 +198,30 @@ def test_batch_graph_replays():
             assert np.array_equal(b["equations"].cpu().numpy(), ye), (rep, b["N"])
 
 
+def test_batch_cost_hints_keep_results():
+    """Buckets started from cost hints (the previous batch's steps_out, and a
+    reversed version of them) give the unhinted batch's results bit for bit,
+    eagerly and as graph replays; a hinted graph is not replayed for a call
+    without hints."""
+    torch = torch_cuda()
+    bk = _mixed_buckets(torch, 2048, seed=11)
+    tend = W.MASS_ACTION_TEND
+    base_total, _ = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, max_attempts=100000)
+    torch.cuda.synchronize()
+    base = [b["equations"].cpu().numpy() for b in bk]
+    hints = [b["steps_out"].clone() for b in bk]
+    for variant, graph in (("steps", False), ("steps", True), ("reversed", True), ("none", True)):
+        for b, h in zip(bk, hints):
+            b["equations"].copy_(b["y0"])
+            b["cost_hint"] = {"steps": h, "reversed": h.max() - h, "none": None}[variant]
+        torch.cuda.synchronize()
+        total, _ = api.integrate_batch(bk, 0.0, tend, 1, MA_ABS, MA_REL, graph=graph, max_attempts=100000)
+        torch.cuda.synchronize()
+        assert total == base_total, (variant, graph)
+        for b, yb in zip(bk, base):
+            assert np.array_equal(b["equations"].cpu().numpy(), yb), (variant, graph, b["N"])
+
+
 def test_batch_scratch_buckets_match_single_calls():
     """Exact mode keeps two N x N matrices per system; at N = 144 they exceed
     LDS and live in the process-wide scratch.  Two such buckets in one batch

@@ -59,7 +59,8 @@ class WindRequestDesc(C.Structure):
 
 class WindBucket(C.Structure):
     _fields_ = [("net", C.c_void_p), ("constants", C.c_void_p), ("equations", C.c_void_p), ("nsys", C.c_int),
-                ("const_stride", C.c_int), ("steps_out", C.c_void_p), ("status_out", C.c_void_p)]
+                ("const_stride", C.c_int), ("steps_out", C.c_void_p), ("status_out", C.c_void_p),
+                ("cost_hint", C.c_void_p)]
 
 
 class WindError(RuntimeError):
@@ -317,12 +318,13 @@ def integrate_batch(buckets, tnow, tend, n_integration_steps, absolute, relative
                     stream=None, max_attempts=0):
     """Integrates a mixed batch.  `buckets` is a list of dicts with keys net (Network),
     constants, equations (device tensors or pointers), nsys, and optional const_stride,
-    steps_out, status_out.  Returns (total_substeps, batch_ms)."""
+    steps_out, status_out, cost_hint.  Returns (total_substeps, batch_ms)."""
     lib = load()
     arr = (WindBucket * len(buckets))()
     for i, b in enumerate(buckets):
         arr[i] = WindBucket(b["net"].handle, _ptr(b["constants"]), _ptr(b["equations"]), b["nsys"],
-                            b.get("const_stride", 0), _ptr(b.get("steps_out")), _ptr(b.get("status_out")))
+                            b.get("const_stride", 0), _ptr(b.get("steps_out")), _ptr(b.get("status_out")),
+                            _ptr(b.get("cost_hint")))
     o = WindOptions()
     lib.wind_options_init(C.byref(o))
     o.method, o.mode, o.controller, o.solver, o.ir_sweeps = method, mode, controller, solver, ir_sweeps

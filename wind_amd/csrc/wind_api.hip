@@ -92,6 +92,7 @@ struct Context {
     std::vector<hipEvent_t> bucket_events;
     hipEvent_t fork = nullptr;
     Pool tune_state, batch_counters;
+    std::vector<Pool> bucket_order;  // per bucket: start order from its cost hints
     // output-step driver
     Pool snapshots, out_counters;
     std::vector<hipEvent_t> out_events;
@@ -1272,6 +1273,20 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
     rc = reserve(c.batch_counters, (size_t)nb * 4 * sizeof(unsigned long long));
     if (rc) return rc;
     unsigned long long *cnt = (unsigned long long *)c.batch_counters.ptr;
+    // Start orders from the buckets' cost hints: buffers reserved here, before
+    // anything is enqueued or captured; the ordering kernels run on each
+    // bucket's stream ahead of its integration kernel.
+    if (c.bucket_order.size() < (size_t)nb) c.bucket_order.resize(nb);
+    std::vector<int32_t *> order_index(nb, nullptr);
+    std::vector<unsigned *> order_counts(nb, nullptr);
+    for (int i : order) {
+        if (!buckets[i].cost_hint || !queue_kernel(buckets[i].net, o)) continue;
+        const size_t index_bytes = ((size_t)buckets[i].nsys * 4 + 255) & ~(size_t)255;
+        rc = reserve(c.bucket_order[i], index_bytes + kHintClasses * sizeof(unsigned));
+        if (rc) return rc;
+        order_index[i] = (int32_t *)c.bucket_order[i].ptr;
+        order_counts[i] = (unsigned *)((char *)c.bucket_order[i].ptr + index_bytes);
+    }
 
     auto enqueue = [&]() -> int {
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, cnt, nb * 4);
@@ -1280,6 +1295,10 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
             hipStream_t bs = c.bucket_streams[stream_of[i]];
             WIND_HIP(hipStreamWaitEvent(bs, c.fork, 0));
             IntegrateArgs a = bucket_args(buckets[i], tnow, tend, nfix, ABSOLUTE, RELATIVE, o, cnt + 4 * i);
+            if (order_index[i]) {
+                wind_order_by_hint(buckets[i].cost_hint, buckets[i].nsys, order_index[i], order_counts[i], bs);
+                a.index = order_index[i];
+            }
             int r = launch_bucket(c, buckets[i].net, a, o, sc[i], bs);
             if (r) return r;
             WIND_HIP(hipEventRecord(c.bucket_events[i], bs));
@@ -1305,6 +1324,8 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
             push((uint64_t)(uintptr_t)b.equations);
             push((uint64_t)(uintptr_t)b.steps_out);
             push((uint64_t)(uintptr_t)b.status_out);
+            push((uint64_t)(uintptr_t)b.cost_hint);
+            push((uint64_t)(uintptr_t)order_index[i]);
             push(((uint64_t)(uint32_t)b.nsys << 32) | (uint32_t)b.const_stride);
             push(((uint64_t)(uint32_t)sc[i].solver << 48) | ((uint64_t)(uint32_t)sc[i].block << 32) |
                  ((uint64_t)(uint32_t)sc[i].ir_sweeps << 24) | ((uint64_t)(uint32_t)sc[i].pivot << 20) |
