@@ -98,9 +98,13 @@ def rel_err(y, truth, floor=1e-3):
 def _acc_stats(e):
     if len(e) == 0:
         return None
+    # with infinite errors in the sample, quantiles without interpolation
+    # (interpolating between inf values gives nan)
+    qm = "linear" if np.isfinite(e).all() else "higher"
     return {"sample": int(len(e)), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
-            "median_rel_err": float(np.median(e)), "p90_rel_err": float(np.quantile(e, 0.9)),
-            "p99_rel_err": float(np.quantile(e, 0.99)), "max_rel_err": float(np.max(e)),
+            "median_rel_err": float(np.quantile(e, 0.5, method=qm)),
+            "p90_rel_err": float(np.quantile(e, 0.9, method=qm)),
+            "p99_rel_err": float(np.quantile(e, 0.99, method=qm)), "max_rel_err": float(np.max(e)),
             "nonfinite": int(np.sum(~np.isfinite(e)))}
 
 
