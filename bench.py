@@ -26,7 +26,8 @@ timed on this host's cores.  Beside the headline: the mixed-N mass-action
 workload (2^18 systems, N in {5, 10, 32, 64, 157}) in one autotuned,
 graph-captured batch with its accuracy against FP64 and the plain
 one-workgroup-per-system kernel's rate; the N = 157 dense kernel against the
-FP32 MFMA peak; and the request server under Poisson arrivals (p50/p99).
+FP32 MFMA peak on a sample and on all 65,536 systems of that workload; and the
+request server under Poisson arrivals (p50/p99).
 
     python bench.py [--steps K] [--warmup W] [--mode fast|exact] [--gpus 1]
 """
@@ -377,6 +378,38 @@ def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
     return out
 
 
+def dense_full(torch, api, W):
+    """The whole N = 157 dense workload (all its systems) in one call with the
+    default solver: systems/s, and the final-state error of the 32 systems
+    with FP64 truth (tests/golden/truth_mass_action.npz) taken from this run."""
+    dev = torch.device("cuda:0")
+    nsys = W.DENSE_NSYS
+    rx, k, y0, sc = W.dense_workload(nsys)
+    net = api.Network.mass_action(W.DENSE_N, rx, k)
+    y = torch.from_numpy(y0).to(dev)
+    c = torch.from_numpy(sc).to(dev)
+    st = torch.zeros(nsys, dtype=torch.int32, device=dev)
+    tot, kms = api.integrate(net, 0.0, W.MASS_ACTION_TEND, 1, c, y, nsys, MA_ABS, MA_REL, mode=api.MODE_FAST,
+                             device_pointers=True, controller=api.CTRL_ADAPTIVE, want_kernel_ms=True,
+                             max_attempts=100000, status_out=st)
+    out = {"workload": f"N = 157 mass-action, all {nsys} systems, t in [0, {W.MASS_ACTION_TEND:g}], SIE, adaptive "
+                       f"controller abs {MA_ABS:g} rel {MA_REL:g}",
+           "kernel": "ma_dense_reg_integrate<10>", "systems_per_s": nsys / (kms / 1e3), "kernel_ms": kms,
+           "substeps": int(tot), "status_ok": int((st == 0).sum())}
+    F = dense_flops(W.DENSE_N)
+    ach = F * float(tot) / (kms / 1e3) / 1e12
+    out.update({"achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS})
+    tp = os.path.join(ROOT, "tests", "golden", "truth_mass_action.npz")
+    if os.path.exists(tp):
+        T = np.load(tp)
+        if float(T["tend"]) == W.MASS_ACTION_TEND:
+            sel = T["dense_157_idx"]
+            e = ma_rel_err(y.cpu().numpy().reshape(-1, W.DENSE_N)[sel].astype(np.float64), T["dense_157_y"])
+            out["accuracy"] = {"sample": int(len(e)), "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3)),
+                               "median_rel_err": float(np.median(e)), "max_rel_err": float(e.max())}
+    return out
+
+
 def dense_roofline(torch, api, W, nsys):
     """The N = 157 dense workload alone on a sample: algorithmic flops per
     second of the dense kernel against the FP32 MFMA peak."""
@@ -584,6 +617,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--dense-nsys", type=int, default=4096)
     ap.add_argument("--no-dense", action="store_true", help="skip the N = 157 dense roofline and precision forms")
+    ap.add_argument("--no-dense-full", action="store_true",
+                    help="skip the run of all 65,536 systems of the N = 157 workload (about 35 s)")
     ap.add_argument("--hint-steps", type=int, default=3,
                     help="timed steps of the headline with the previous step's sub-step counts as cost hints")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -794,6 +829,9 @@ def main():
     progress("mixed batch done")
     dense = None if a.no_dense else dense_roofline(torch, api, W, a.dense_nsys)
     progress("dense N = 157 done")
+    if dense is not None and not a.no_dense_full:
+        dense["full_workload"] = dense_full(torch, api, W)
+        progress("dense N = 157, all systems, done")
     precision = None if a.no_dense else dense_precision(torch, api, W, 1024)
     progress("precision forms done")
     serve = None if a.no_serve else serving(api, W, a.serve_seconds, a.serve_load)

@@ -101,10 +101,10 @@ def test_noalloc_exact_reference_controller():
             api.CTRL_REFERENCE, stride=4)
 
 
-@pytest.mark.parametrize("N", [10, 32])
+@pytest.mark.parametrize("N", [10, 32, 64])
 def test_noalloc_mass_action_dense(N):
-    """Mass-action networks in fast mode (the dense LDS/MFMA solver, work
-    queue in the scratch array)."""
+    """Mass-action networks in fast mode (the dense LDS/MFMA solver; at
+    N = 64 the register-resident one; work queue in the scratch array)."""
     rx, k = W.mass_action_network(N, seed=N)
     y0, sc = W.mass_action_systems(600, N, seed=40 + N, scale_range=(0.5, 2.0))
     net = api.Network.mass_action(N, rx, k)
