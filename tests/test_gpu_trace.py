@@ -226,9 +226,12 @@ def test_fast_lane_attempts_hhe6(method, tol, cap):
     attempt_parity(method, O.hhe6(p), c, n, tr, t, ystart, tol, f"hhe6 fast per-attempt {('SIE', 'RK2')[method]}")
 
 
-# N = 157 runs on the register-resident solver by default and on the
-# LDS-resident one with FLAG_DENSE_LDS; N = 32, 64 on the LDS-resident one.
-DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (64, 32, 240, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
+# 49 <= N <= 160 runs on the register-resident solver by default (tile
+# counts per dimension 4 .. 10; N = 80, 100, 129 cover 5, 7 and 9, odd
+# counts leaving some of a wave's tile slots unused) and on the LDS-resident
+# one with FLAG_DENSE_LDS; N = 32 on the LDS-resident one.
+DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (64, 32, 240, 0), (80, 16, 200, 0), (100, 12, 160, 0), (129, 8, 150, 0),
+                       (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 
 def dense_case(N, ns):
@@ -295,7 +298,8 @@ def test_fast_lane_replay_hhe6_sie():
     whole_run_parity(api.SIE, O.hhe6(p), c, y0, n, 6, tr, y, tol, "hhe6 fast replay SIE", t)
 
 
-DENSE_CASES = [(32, 96, 0, 0), (64, 32, 240, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
+DENSE_CASES = [(32, 96, 0, 0), (64, 32, 240, 0), (100, 12, 240, 0), (157, 8, 150, 0),
+               (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 
 @pytest.mark.parametrize("N,ns,cap,flags", DENSE_CASES)
@@ -303,8 +307,8 @@ def test_fast_dense_replay(N, ns, cap, flags):
     """The dense LDS/MFMA solver carries its own state through the oracle's
     accepted-step sequence: over the whole interval for N = 32 (the systems
     with at least 100 accepted steps) and over the first 240 / 150 attempts
-    for N = 64 / 157 (at least 100 accepted steps each); floor 1e-6 of the
-    system's total abundance."""
+    for N = 64, 100 / 157 (at least 100 accepted steps each); floor 1e-6 of
+    the system's total abundance."""
     rx, k, y0, sc = dense_case(N, ns)
     tend = W.MASS_ACTION_TEND
     onet = O.mass_action(N, rx, k)
