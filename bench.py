@@ -25,7 +25,9 @@ denominator), and the oracle (CPU restatement of the reference c_baseline)
 timed on this host's cores.  Beside the headline: the mixed-N mass-action
 workload (2^18 systems, N in {5, 10, 32, 64, 157}) in one autotuned,
 graph-captured batch with its accuracy against FP64 and the plain
-one-workgroup-per-system kernel's rate; the N = 157 dense kernel against the
+one-workgroup-per-system kernel's rate; WIND's own published benchmark (Katz96
+tiled to N = 5 .. 250, 80 systems, through the drop-in) beside its published
+time-to-solution fits; the N = 157 dense kernel against the
 FP32 MFMA peak on a sample and on all 65,536 systems of that workload; and the
 request server under Poisson arrivals (p50/p99).
 
@@ -214,6 +216,78 @@ def nr_test_c1(W, reps=5):
             "bit_identical_to_oracle": bool(np.array_equal(y.view(np.uint32), yr.view(np.uint32))),
             "accuracy_vs_closed_form": {"max_rel_err": float(e.max()), "median_rel_err": float(np.median(e)),
                                         "frac_rel_err_le_1e-3": float(np.mean(e <= 1e-3))}}
+
+
+# ----------------------------------------- the reference's published benchmark
+# Least-squares fits of WIND's Ntile-scaling time to solution (tts = b + a N for
+# RK2, b + a N^3 for SIE), python/ntbks/NtileScaling.ipynb:173,185,245-254:
+# hardware -> method -> (b seconds, a seconds per N or per N^3).
+WIND_PUBLISHED_TTS = {
+    "Tesla K20x": {"rk2": (0.208, 117.2e-6), "sie": (0.269, 32.6e-6)},
+    "Tesla K40m": {"rk2": (0.153, 393.9e-6), "sie": (0.214, 5.5e-6)},
+    "GTX 1050 Ti": {"rk2": (0.061, 114.5e-6), "sie": (0.088, 3.3e-6)},
+    "K40m node host CPU": {"rk2": (0.004, 118.2e-6), "sie": (0.004, 3.9e-6)},
+    "laptop host CPU": {"rk2": (0.001, 39.1e-6), "sie": (0.003, 2.7e-6)},
+}
+
+
+def wind_published_katz96(W, reps=3, cpu_max_n=100):
+    """WIND's own published benchmark, run the way its harness runs it: Katz96
+    tiled Ntile times (N = 5 Ntile), the 4 systems of its smoke-test input
+    repeated 20 times (80 systems), t in [0, 200] yr, one output step,
+    n_integration_steps = 1, abs = rel = 5e-3 (Katz96_neqntile.<T>_
+    nsystemtile.20_fixed.1_abs.5e3_rel.5e3).  Time to solution of one call of
+    the drop-in cudaIntegrateSystem (sie.so / rk2.so of the generated network:
+    exact mode, host arrays in and out), median of `reps` after a warm-up call,
+    beside the published fits evaluated at the same N; the oracle on one core
+    for N <= cpu_max_n, and the GPU result checked against it bit for bit."""
+    import ctypes as C
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    rows = []
+    for T in W.KATZ96_SCALING_NTILES:
+        name = "katz96_gen" if T == 1 else f"katz96_t{T}"
+        d = os.path.join(ROOT, "wind_amd", "lib", "gen", name)
+        if not os.path.exists(os.path.join(d, "sie.so")):
+            continue
+        N = 5 * T
+        y0, c = W.katz96_scaling_systems(T)
+        ns = len(y0) // N
+        row = {"ntile": T, "N": N, "systems": ns}
+        for meth, lib in (("sie", "sie.so"), ("rk2", "rk2.so")):
+            fn = getattr(C.CDLL(os.path.join(d, lib)), "_Z19cudaIntegrateSystemffiPfS_iiff")
+            fn.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                           C.c_float]
+            ts, y = [], None
+            for _ in range(reps + 1):
+                y = y0.copy()
+                t = time.perf_counter()
+                fn(0.0, 200.0, 1, c.ctypes.data, y.ctypes.data, ns, N, 5e-3, 5e-3)
+                ts.append(time.perf_counter() - t)
+            r = {"tts_ms": float(np.median(ts[1:])) * 1e3}
+            expo = 3 if meth == "sie" else 1
+            pub = {hw: (f[meth][0] + f[meth][1] * N ** expo) * 1e3 for hw, f in WIND_PUBLISHED_TTS.items()}
+            r["published_fit_tts_ms"] = pub
+            best_gpu = min(v for hw, v in pub.items() if "CPU" not in hw)
+            r["speedup_vs_best_published_gpu"] = best_gpu / r["tts_ms"]
+            if N <= cpu_max_n:
+                yr = y0.copy()
+                t = time.perf_counter()
+                O.gold_integrate(O.SIE if meth == "sie" else O.RK2, O.generated(name), 0.0, 200.0, 1, c, yr, ns,
+                                 5e-3, 5e-3, const_stride=10)
+                r["oracle_one_core_ms"] = (time.perf_counter() - t) * 1e3
+                r["bit_identical_to_oracle"] = bool(np.array_equal(y.view(np.uint32), yr.view(np.uint32)))
+            row[meth] = r
+        rows.append(row)
+        progress(f"WIND published benchmark: N = {N} done (SIE {row['sie']['tts_ms']:.2f} ms, "
+                 f"RK2 {row['rk2']['tts_ms']:.2f} ms)")
+    return {"workload": "Katz96 tiled Ntile times (N = 5 Ntile), 80 systems (the 4 smoke-test systems x 20), "
+                        "t in [0, 200] yr, n_integration_steps = 1, abs = rel = 5e-3, one cudaIntegrateSystem "
+                        "call through the drop-in (exact mode, host arrays)",
+            "source": "python/ntbks/NtileScaling.ipynb (published fits tts = b + a N for RK2, b + a N^3 for SIE)",
+            "rows": rows}
 
 
 # ------------------------------------------------------- hhe6 RK2 (config 2)
@@ -610,6 +684,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-rk2", action="store_true", help="skip the RK2 leg of the hhe6 workload")
     ap.add_argument("--no-c1", action="store_true", help="skip the NR_test (1024 systems) GPU and CPU row")
+    ap.add_argument("--no-wind-published", action="store_true",
+                    help="skip WIND's published Katz96 Ntile-scaling benchmark (N = 5 .. 250)")
     ap.add_argument("--no-generated", action="store_true",
                     help="mixed batch: table-driven networks for every N (no generated plugins)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="open-loop seconds per offered load")
@@ -824,6 +900,8 @@ def main():
     progress("rk2 leg done")
     c1 = None if a.no_c1 else nr_test_c1(W)
     progress("NR_test row done")
+    wind_pub = None if a.no_wind_published else wind_published_katz96(W)
+    progress("WIND published benchmark done")
     mixed = None if a.no_mixed else mixed_batch(torch, api, W, a.mixed_nsys, a.mixed_steps, not a.no_plain,
                                                 not a.no_generated)
     progress("mixed batch done")
@@ -859,7 +937,7 @@ def main():
         "accuracy": acc, "with_cost_hint": hinted, "looser_tolerances": sweep, "roofline": roofline, "host_buffers_pcie_inclusive": host_path,
         "reference_algorithm": ref_alg, "plain_gpu": plain, "rk2": rk2, "nr_test_1024": c1,
         "cpu_baseline": cpu, "cpu_same_algorithm": cpu_same,
-        "mixed_batch": mixed, "dense_157": dense, "dense_157_precision": precision, "serving": serve,
+        "wind_published_katz96": wind_pub, "mixed_batch": mixed, "dense_157": dense, "dense_157_precision": precision, "serving": serve,
     }
     print(json.dumps(line))
 

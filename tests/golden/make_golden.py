@@ -25,12 +25,9 @@ import oracle_lib as O  # noqa: E402
 OUT = os.path.join(HERE, "c_baseline_kats.npz")
 
 # cuda/debug/inputKatz96_1_1_1.h:1-10 (inputs of the reference's smoke test)
-KATZ_FIXTURE_EQ = np.array([0, 1.000e+00, 0, 9.929e-02, 0] * 4, np.float32)
-KATZ_FIXTURE_C = np.array(
-    [0, 1.386e-03, 4.187e-02, 0, 1.165e-04, 0, 5.355e-07, 2.534e-02, 0, 1.675e-01,
-     0, 1.386e-03, 5.271e-02, 0, 1.165e-04, 0, 5.355e-07, 3.190e-02, 0, 2.108e-01,
-     0, 1.386e-03, 3.563e-02, 0, 1.165e-04, 0, 5.355e-07, 2.189e-02, 0, 1.425e-01,
-     0, 1.386e-03, 4.485e-02, 0, 1.165e-04, 0, 5.355e-07, 2.756e-02, 0, 1.794e-01], np.float32)
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from wind_amd.networks import KATZ96_FIXTURE_C as KATZ_FIXTURE_C  # noqa: E402
+from wind_amd.networks import KATZ96_FIXTURE_Y as KATZ_FIXTURE_EQ  # noqa: E402
 
 
 def sha(a):

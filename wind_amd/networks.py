@@ -189,6 +189,30 @@ def nr_test_spec(name: str = "nr_test_gen", ntile: int = 1) -> dict:
 # Katz96 tiled as in the reference's Ntile scaling study (NtileScaling.ipynb,
 # neqn_test.sh): 50 tiles = 250 equations.
 KATZ96_NTILE = 50
+# Katz96 tilings of the Ntile-scaling benchmark (python/ntbks/NtileScaling.ipynb
+# runs Ntile in 1 .. 50, N = 5 .. 250): the ones the build generates.
+KATZ96_SCALING_NTILES = (1, 5, 10, 20, 50)
+
+# The reference's Katz96 smoke-test inputs (cuda/debug/inputKatz96_1_1_1.h:
+# 4 systems, 5 species, 10 rate constants per system in 1/yr; t in [0, 200],
+# n_integration_steps = 1, abs = rel = 5e-3).
+KATZ96_FIXTURE_Y = np.array([0, 1.000e+00, 0, 9.929e-02, 0] * 4, np.float32)
+KATZ96_FIXTURE_C = np.array(
+    [0, 1.386e-03, 4.187e-02, 0, 1.165e-04, 0, 5.355e-07, 2.534e-02, 0, 1.675e-01,
+     0, 1.386e-03, 5.271e-02, 0, 1.165e-04, 0, 5.355e-07, 3.190e-02, 0, 2.108e-01,
+     0, 1.386e-03, 3.563e-02, 0, 1.165e-04, 0, 5.355e-07, 2.189e-02, 0, 1.425e-01,
+     0, 1.386e-03, 4.485e-02, 0, 1.165e-04, 0, 5.355e-07, 2.756e-02, 0, 1.794e-01], np.float32)
+
+
+def katz96_scaling_systems(ntile: int, nsystem_tile: int = 20):
+    """Inputs of the Ntile-scaling runs (Katz96_neqntile.<T>_nsystemtile.20_
+    fixed.1_abs.5e3_rel.5e3, python/ntbks/NtileScaling.ipynb): the fixture's
+    4 systems repeated nsystem_tile times, each system's 5 species repeated
+    ntile times (the tiled network's block-diagonal copies, ode_base.py:215-242),
+    10 constants per system shared by its tiles.  Returns (y0, constants)."""
+    ys = np.tile(KATZ96_FIXTURE_Y.reshape(4, 5), (nsystem_tile, ntile))
+    cs = np.tile(KATZ96_FIXTURE_C.reshape(4, 10), (nsystem_tile, 1))
+    return ys.reshape(-1).copy(), cs.reshape(-1).copy()
 
 
 def generated_specs():
@@ -197,7 +221,8 @@ def generated_specs():
     network with one- and two-body reactions and two constants per system
     ("demo_decay"), that network tiled three times ("demo_decay_t3"), and the
     reference's networks from their expressions: Katz96 ("katz96_gen"), Katz96
-    tiled 4 and 50 times ("katz96_t4", N = 20; "katz96_t50", N = 250) and
+    tiled 4, 5, 10, 20 and 50 times ("katz96_t4", N = 20, ... "katz96_t50",
+    N = 250; the tilings of the Ntile-scaling benchmark) and
     NR_test ("nr_test_gen"), and a small system with a zero leading pivot
     ("pivot_demo", "pivot_demo_t10")."""
     from wind_amd import codegen
@@ -220,6 +245,9 @@ def generated_specs():
     specs.append({**demo, "name": "demo_decay_t3", "ntile": 3})
     specs.append(katz96_spec())
     specs.append(katz96_spec("katz96_t4", 4))
+    for T in KATZ96_SCALING_NTILES:
+        if T not in (1, 4, KATZ96_NTILE):
+            specs.append(katz96_spec(f"katz96_t{T}", T))
     specs.append(katz96_spec(f"katz96_t{KATZ96_NTILE}", KATZ96_NTILE))
     specs.append(nr_test_spec())
     specs.append(pivot_demo_spec())
