@@ -88,7 +88,7 @@ inline int gen_block_shape(int nsys, int num_cu, float share, int grid_cap, GenB
         return WIND_ERR_DEVICE;
     const size_t base = sizeof(float) * (size_t)gen_block_base_floats<Net>();
     const size_t mats = sizeof(float) * (FAST ? 1 : 2) * (size_t)N * N;
-    s->T = N <= 64 ? 64 : 256;
+    s->T = block_threads(N, nsys, num_cu);
     s->in_lds = base + mats + 256 <= (size_t)max_lds;
     s->lds = base + (s->in_lds ? mats : 0);
     auto kfn = gen_block_integrate<Net, METHOD, FAST, PIVOT>;

@@ -71,8 +71,20 @@ __device__ inline void ma_eval(const MassActionDevice &net, const float *yv, flo
 }
 
 // Exact Gauss-Jordan inverse and mat-vec; on return g = inverse * f.
+// Threads of a one-workgroup-per-system launch: 64 (one wave) for N <= 64
+// when the systems fill the device, four waves otherwise, so that a few
+// systems (latency-bound) or large ones spread each elimination step over
+// more lanes.  Results do not depend on it.
+inline int block_threads(int N, int nsys, int num_cu) { return (N > 64 || nsys <= 4 * num_cu) ? 256 : 64; }
+
 __device__ inline void block_gje_exact(int N, MALayout &L) {
     const int T = blockDim.x, t = threadIdx.x;
+    // Elimination loops: groups of TX lanes (the power of two >= N, at most
+    // 64) walk a row's columns, the groups walk the rows, so that no thread
+    // divides by N.  Every element's arithmetic is the reference's whatever
+    // thread does it.
+    const int lg = N <= 8 ? 3 : N <= 16 ? 4 : N <= 32 ? 5 : 6, TX = 1 << lg;
+    const int tx = t & (TX - 1), ty = t >> lg, TY = T >> lg;
     float *M = L.M, *Inv = L.Inv;
     for (int e = t; e < N * N; e += T) Inv[e] = (e % (N + 1) == 0) ? 1.0f : 0.0f;
     __syncthreads();
@@ -85,14 +97,14 @@ __device__ inline void block_gje_exact(int N, MALayout &L) {
                 Inv[r * N + c] = Inv[r * N + c] / s;
         }
         __syncthreads();
-        const int rows = N - 1 - r;
-        for (int idx = t; idx < rows * N; idx += T) {
-            const int q = r + 1 + idx / N, c = idx % N;
+        for (int q = r + 1 + ty; q < N; q += TY) {
             const float sq = M[q * N + r];
-            if (c > r)
-                M[q * N + c] = M[q * N + c] - sq * M[r * N + c];
-            else
-                Inv[q * N + c] = Inv[q * N + c] - sq * Inv[r * N + c];
+            for (int c = tx; c < N; c += TX) {
+                if (c > r)
+                    M[q * N + c] = M[q * N + c] - sq * M[r * N + c];
+                else
+                    Inv[q * N + c] = Inv[q * N + c] - sq * Inv[r * N + c];
+            }
         }
         __syncthreads();
     }
@@ -100,9 +112,9 @@ __device__ inline void block_gje_exact(int N, MALayout &L) {
         const int b = N - 1 - r;
         // rows bq < b, processed in the reference's order bq = b-1, b-2, ...;
         // the rows are independent, so any assignment to threads is exact.
-        for (int idx = t; idx < b * N; idx += T) {
-            const int bq = idx / N, c = idx % N;
-            Inv[bq * N + c] = Inv[bq * N + c] - M[bq * N + b] * Inv[b * N + c];
+        for (int bq = ty; bq < b; bq += TY) {
+            const float m = M[bq * N + b];
+            for (int c = tx; c < N; c += TX) Inv[bq * N + c] = Inv[bq * N + c] - m * Inv[b * N + c];
         }
         __syncthreads();
     }

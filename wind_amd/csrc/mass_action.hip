@@ -215,7 +215,7 @@ auto block_kernel(bool fast, bool pivot) {
 template <int METHOD>
 int block_shape(int num_cu, const MassActionDevice &d, const MassActionPlan &plan, bool fast, const SolverChoice &sc,
                 int nsys, BlockShape *s) {
-    s->T = d.N <= 64 ? 64 : 256;
+    s->T = block_threads(d.N, nsys, num_cu);
     s->in_lds = plan.in_lds;
     s->lds = plan.lds;
     int max_lds = 0, dev = 0;
