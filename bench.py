@@ -794,7 +794,8 @@ def main():
         "bound": "valu", "achieved": flops / kernel_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": flops / kernel_s / 1e12 / PEAK_FP32_TFLOPS, "traffic": None,
         "kernel": kname, "kernel_ms": kernel_s * 1e3,
-        "kernel_ms_includes": "the start-order pre-pass (lane_activity, hint_* kernels, ~0.15 ms)",
+        "kernel_ms_includes": "the start-order pre-pass (lane_probe: 16 attempts per system, lane_activity_hints, "
+                              "hint_* kernels)",
         "flops_per_substep": F_SUBSTEP,
         "flops_source": "algorithmic (2 network evaluations and 3 dense solves per attempt)",
         "frac_of_unpacked_fp32_peak": flops / kernel_s / 1e12 / PEAK_FP32_UNPACKED_TFLOPS,

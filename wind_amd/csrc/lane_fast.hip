@@ -69,6 +69,56 @@ __global__ void __launch_bounds__(256) lane_activity(IntegrateArgs a, typename N
     if (__lane_id() == 0 && best) atomicMax(xmax, best);
 }
 
+// Start-order estimate from a short probe: the first kProbeAttempts
+// attempts of each system from its initial state (lane_fast_kernel's
+// attempts; the state is then discarded), extrapolated to the end of the
+// interval: attempts so far + (tend - t) / h x attempts per accepted one,
+// with h the controller's next step (a system that finishes within the probe
+// counts its attempts).  Float bits to xbits[s], the largest to *xmax, as
+// lane_activity.
+constexpr int kProbeAttempts = 16;
+
+template <class Net, int METHOD, class M>
+__global__ void __launch_bounds__(256) lane_probe(IntegrateArgs a, typename Net::Params p, unsigned *xbits,
+                                                   unsigned *xmax) {
+    constexpr int N = Net::N;
+    constexpr int NC = Net::NCONST;
+    unsigned best = 0u;
+    for (int sid = blockIdx.x * blockDim.x + threadIdx.x; sid < a.nsys; sid += gridDim.x * blockDim.x) {
+        LaneSys<N> s;
+        float c[NC];
+        load_state<N>(a.equations + (size_t)sid * N, s.y);
+#pragma unroll
+        for (int k = 0; k < NC; k++) c[k] = a.constants[(size_t)sid * a.const_stride + k];
+        s.tnow = a.tnow;
+        s.h = a.h0;
+        s.unsolved = 0;
+        s.attempts = 0;
+        s.status = WIND_STATUS_OK;
+        int accepted = 0;
+        bool done = false;
+        for (int k = 0; k < kProbeAttempts && !done; k++) {
+            const float t0 = s.tnow;
+            done = attempt<Net, METHOD, M>(p, a, c, s);
+            accepted += s.tnow != t0 ? 1 : 0;
+        }
+        float x = (float)s.attempts;
+        if (!done)
+            x += (a.tend - s.tnow) / fmaxf(s.h, 1.0e-30f) * ((float)s.attempts / (float)(accepted > 0 ? accepted : 1));
+        if (!(x >= 0.0f)) x = 0.0f;                   // NaN
+        if (x > 3.0e38f) x = 3.0e38f;                 // inf
+        const unsigned b = __float_as_uint(x);
+        xbits[sid] = b;
+        best = b > best ? b : best;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(best, off);
+        best = o > best ? o : best;
+    }
+    if (__lane_id() == 0 && best) atomicMax(xmax, best);
+}
+
 // Turns the estimates into cost hints whose classes (four per octave, as
 // the start-order sort forms them) keep 29 octaves below the largest: an
 // estimate e octaves below it gets class 124 - floor(4 e) (at least 9), and
@@ -94,6 +144,19 @@ int launch_lane_activity(int num_cu, const typename Net::Params &p, const Integr
     unsigned *xmax = scratch, *xbits = scratch + 64;
     const int grid = std::min(num_cu * 8, (a.nsys + 255) / 256);
     hipLaunchKernelGGL(lane_activity<Net>, dim3(grid), dim3(256), 0, st, a, p, xbits, xmax);
+    hipLaunchKernelGGL(lane_activity_hints, dim3(grid), dim3(256), 0, st, xbits, a.nsys, xmax, hint);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WIND_OK : wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
+}
+
+template <class Net, int METHOD>
+int launch_lane_probe(int num_cu, const typename Net::Params &p, const IntegrateArgs &a, int32_t *hint,
+                      unsigned *scratch, hipStream_t st) {
+    // scratch: [0] the largest estimate, then one word per system
+    unsigned *xmax = scratch, *xbits = scratch + 64;
+    const int grid = std::min(num_cu * 8, (a.nsys + 255) / 256);
+    auto k = a.pivot ? lane_probe<Net, METHOD, MathFastPivot> : lane_probe<Net, METHOD, MathFast>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p, xbits, xmax);
     hipLaunchKernelGGL(lane_activity_hints, dim3(grid), dim3(256), 0, st, xbits, a.nsys, xmax, hint);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? WIND_OK : wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
@@ -125,6 +188,10 @@ int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateA
 #define WIND_LANE_FAST(NET)                                                                                   \
     template int launch_lane_activity<NET>(int, const NET::Params &, const IntegrateArgs &, int32_t *,         \
                                            unsigned *, hipStream_t);                                          \
+    template int launch_lane_probe<NET, WIND_SIE>(int, const NET::Params &, const IntegrateArgs &, int32_t *, \
+                                                  unsigned *, hipStream_t);                                   \
+    template int launch_lane_probe<NET, WIND_RK2>(int, const NET::Params &, const IntegrateArgs &, int32_t *, \
+                                                  unsigned *, hipStream_t);                                   \
     template int launch_lane_fast<NET, WIND_SIE>(int, const NET::Params &, const IntegrateArgs &,             \
                                                  const SolverChoice &, hipStream_t);                          \
     template int launch_lane_fast<NET, WIND_RK2>(int, const NET::Params &, const IntegrateArgs &,             \

@@ -249,20 +249,26 @@ bool queue_kernel(const wind_network *net, const wind_options &o) { return wind_
 // once, and smaller batches start every system immediately.
 constexpr int kAutoOrderMin = 1 << 17;
 
+// Start-order estimate without caller hints: a short probe integration of
+// every system (true) or the initial state's fastest rate of change (false).
+constexpr bool kAutoOrderProbe = true;
+
 template <class Net>
-int activity_hints_for(int num_cu, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
+int activity_hints_for(int num_cu, const wind_network *net, const IntegrateArgs &a, int method, int32_t *hint,
                        unsigned *scratch, hipStream_t st) {
     typename Net::Params p{};
     if constexpr (sizeof(typename Net::Params) > 1) std::memcpy(&p, net->params.data(), sizeof(p));
-    return launch_lane_activity<Net>(num_cu, p, a, hint, scratch, st);
+    if (!kAutoOrderProbe) return launch_lane_activity<Net>(num_cu, p, a, hint, scratch, st);
+    return method == WIND_SIE ? launch_lane_probe<Net, WIND_SIE>(num_cu, p, a, hint, scratch, st)
+                              : launch_lane_probe<Net, WIND_RK2>(num_cu, p, a, hint, scratch, st);
 }
 
-int activity_hints(int num_cu, const wind_network *net, const IntegrateArgs &a, int32_t *hint,
+int activity_hints(int num_cu, const wind_network *net, const IntegrateArgs &a, int method, int32_t *hint,
                    unsigned *scratch, hipStream_t st) {
     switch (net->builtin) {
-    case WIND_NET_NR_TEST: return activity_hints_for<NetNRTest>(num_cu, net, a, hint, scratch, st);
-    case WIND_NET_KATZ96: return activity_hints_for<NetKatz96>(num_cu, net, a, hint, scratch, st);
-    case WIND_NET_HHE6: return activity_hints_for<NetHHe6>(num_cu, net, a, hint, scratch, st);
+    case WIND_NET_NR_TEST: return activity_hints_for<NetNRTest>(num_cu, net, a, method, hint, scratch, st);
+    case WIND_NET_KATZ96: return activity_hints_for<NetKatz96>(num_cu, net, a, method, hint, scratch, st);
+    case WIND_NET_HHE6: return activity_hints_for<NetHHe6>(num_cu, net, a, method, hint, scratch, st);
     default: return fail(WIND_ERR_UNSUPPORTED, "no work estimate for this network");
     }
 }
@@ -867,7 +873,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
         if (!rc) rc = reserve(sl->activity, ((size_t)Nsystems + 64) * 4);
         if (rc) return rc;
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)sl->activity.ptr, 32);
-        rc = activity_hints(c.num_cu, net, a, (int32_t *)sl->hint.ptr, (unsigned *)sl->activity.ptr, st);
+        rc = activity_hints(c.num_cu, net, a, opts.method, (int32_t *)sl->hint.ptr, (unsigned *)sl->activity.ptr, st);
         if (!rc) rc = order_by_hint(sl->order, (const int32_t *)sl->hint.ptr, Nsystems, st, &a.index);
         if (rc) return rc;
     }
@@ -1028,7 +1034,7 @@ extern "C" int wind_integrate_noalloc(const wind_network *net, float tnow, float
         a.index = order;
     } else if (auto_order(net, opts, Nsystems)) {
         wind_zero_counters((unsigned long long *)activity, 32, st);
-        rc = activity_hints(d.num_cu, net, a, hint, activity, st);
+        rc = activity_hints(d.num_cu, net, a, opts.method, hint, activity, st);
         if (rc) return rc;
         wind_order_by_hint(hint, Nsystems, order, counts, st);
         WIND_HIP(hipGetLastError());

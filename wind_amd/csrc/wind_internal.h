@@ -101,6 +101,11 @@ int launch_lane_fast(int num_cu, const typename Net::Params &p, const wind::Inte
 template <class Net>
 int launch_lane_activity(int num_cu, const typename Net::Params &p, const wind::IntegrateArgs &a, int32_t *hint,
                          unsigned *scratch, hipStream_t st);
+// The same from a short probe integration of each system (its first
+// attempts, extrapolated to the end of the interval).
+template <class Net, int METHOD>
+int launch_lane_probe(int num_cu, const typename Net::Params &p, const wind::IntegrateArgs &a, int32_t *hint,
+                      unsigned *scratch, hipStream_t st);
 
 // True when launch_mass_action would use the process-wide matrix scratch
 // (matrices too large for LDS): only callers holding the context lock.
