@@ -121,7 +121,7 @@ def test_plugin_descriptor_and_counts(name):
     lib = C.CDLL(os.path.join(GEN, name, "libwind_net.so"))
     lib.wind_plugin_get.restype = C.POINTER(Desc)
     d = lib.wind_plugin_get().contents
-    assert d.abi == 3 and d.N == g["N"] and d.nconst == g["nconst"] and d.nreactions == g["R"]
+    assert d.abi == 4 and d.N == g["N"] and d.nconst == g["nconst"] and d.nreactions == g["R"]
     assert d.name.decode() == name and d.flops_rhs == g["flops_rhs"] and d.flops_jac == g["flops_jac"]
     assert d.launch and d.lockstep_attempt
 

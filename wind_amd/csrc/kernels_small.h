@@ -390,12 +390,32 @@ __device__ inline void lane_integrate_body(const IntegrateArgs &a, const typenam
             need = false;
             active = got < a.nsys;
             if (active) {
-                sid = a.index ? a.index[got] : got;
-                const float *ys = a.equations + (size_t)sid * N;
-                const float *cs = a.constants + (size_t)sid * a.const_stride;
-                load_state<N>(ys, s.y);
+                if (a.records) {
+                    // the system's packed record, read in 16-byte pieces
+                    constexpr int R = lane_record_words(N, NC);
+                    float r[R];
+                    const float4 *rp = reinterpret_cast<const float4 *>(a.records + (size_t)got * R);
 #pragma unroll
-                for (int k = 0; k < NC; k++) c[k] = cs[k];
+                    for (int v = 0; v < R / 4; v++) {
+                        const float4 x = rp[v];
+                        r[4 * v] = x.x;
+                        r[4 * v + 1] = x.y;
+                        r[4 * v + 2] = x.z;
+                        r[4 * v + 3] = x.w;
+                    }
+#pragma unroll
+                    for (int i = 0; i < N; i++) s.y[i] = r[i];
+#pragma unroll
+                    for (int k = 0; k < NC; k++) c[k] = r[N + k];
+                    sid = __float_as_int(r[N + NC]);
+                } else {
+                    sid = a.index ? a.index[got] : got;
+                    const float *ys = a.equations + (size_t)sid * N;
+                    const float *cs = a.constants + (size_t)sid * a.const_stride;
+                    load_state<N>(ys, s.y);
+#pragma unroll
+                    for (int k = 0; k < NC; k++) c[k] = cs[k];
+                }
                 s.tnow = a.tnow;
                 s.h = a.h0;
                 s.unsolved = 0;

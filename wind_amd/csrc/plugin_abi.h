@@ -9,7 +9,7 @@
 
 #include "wind_types.h"
 
-#define WIND_PLUGIN_ABI 3
+#define WIND_PLUGIN_ABI 4
 
 struct wind_plugin_desc {
     int abi;          // WIND_PLUGIN_ABI

@@ -71,6 +71,7 @@ struct CallSlot {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Pool constants, equations, steps, status, counters;
     Pool hint, order;  // cost hints copied from the host; start order and class counts
+    Pool records;      // systems packed in start order (lane kernels of compiled-in networks)
     Pool activity;     // initial-state work estimates (start order without caller hints)
     Pool trace;        // per-attempt trace arrays of host-pointer calls
     bool busy = false;
@@ -140,8 +141,10 @@ __global__ void hint_starts(unsigned *count) {
 }
 
 // Each block reserves its chunk's range in every class, then places its
-// systems there.
-__global__ void hint_place(const int32_t *hint, int n, unsigned *start, int32_t *index) {
+// systems there; with pk.rec, it also packs each system's state, constants
+// and id into its record at that position (read in index order, one record
+// written per system).
+__global__ void hint_place(const int32_t *hint, int n, unsigned *start, int32_t *index, PackArgs pk) {
     __shared__ unsigned h[kHintClasses], base[kHintClasses];
     for (int i = threadIdx.x; i < kHintClasses; i += blockDim.x) h[i] = 0;
     __syncthreads();
@@ -156,7 +159,18 @@ __global__ void hint_place(const int32_t *hint, int n, unsigned *start, int32_t 
     __syncthreads();
     for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const int c = hint_class(hint[i]);
-        index[base[c] + atomicAdd(&h[c], 1u)] = i;
+        const unsigned pos = base[c] + atomicAdd(&h[c], 1u);
+        index[pos] = i;
+        if (pk.rec) {
+            // whole 16-byte stores, so that a record is written as full sectors
+            float4 *r = reinterpret_cast<float4 *>(pk.rec + (size_t)pos * pk.R);
+            auto word = [&](int j) {
+                return j < pk.N ? pk.eq[(size_t)i * pk.N + j]
+                       : j < pk.N + pk.NC ? pk.cst[(size_t)i * pk.stride + (j - pk.N)]
+                       : j == pk.N + pk.NC ? __int_as_float(i) : 0.0f;
+            };
+            for (int v = 0; v < pk.R / 4; v++) r[v] = make_float4(word(4 * v), word(4 * v + 1), word(4 * v + 2), word(4 * v + 3));
+        }
     }
 }
 
@@ -230,12 +244,13 @@ int reserve(Pool &p, size_t bytes) {
 
 // Writes the longest-first start order of `n` systems from device hints
 // into pool `order` (the permutation, then the class counts) on stream st.
-int order_by_hint(Pool &order, const int32_t *d_hint, int n, hipStream_t st, const int32_t **index_out) {
+int order_by_hint(Pool &order, const int32_t *d_hint, int n, hipStream_t st, const int32_t **index_out,
+                  const PackArgs *pk = nullptr) {
     const size_t index_bytes = ((size_t)n * 4 + 255) & ~(size_t)255;
     const int rc = reserve(order, index_bytes + kHintClasses * sizeof(unsigned));
     if (rc) return rc;
     int32_t *index = (int32_t *)order.ptr;
-    wind_order_by_hint(d_hint, n, index, (unsigned *)((char *)order.ptr + index_bytes), st);
+    wind_order_by_hint(d_hint, n, index, (unsigned *)((char *)order.ptr + index_bytes), st, pk);
     WIND_HIP(hipGetLastError());
     *index_out = index;
     return WIND_OK;
@@ -858,6 +873,22 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
     // caller's cost hints, or (fast mode, compiled-in networks, batches
     // larger than the device runs at once) from an estimate of each
     // system's work made from its initial state.
+    // Lane kernels of the compiled-in networks also get the systems packed
+    // in that order (records), so that their lanes read contiguously.
+    PackArgs pk;
+    const bool pack = net->builtin == WIND_NET_NR_TEST || net->builtin == WIND_NET_KATZ96 ||
+                      net->builtin == WIND_NET_HHE6;
+    if (pack && (opts.cost_hint || auto_order(net, opts, Nsystems)) && queue_kernel(net, opts)) {
+        pk.eq = a.equations;
+        pk.cst = a.constants;
+        pk.N = Neqn;
+        pk.NC = net->nconst;
+        pk.stride = a.const_stride;
+        pk.R = lane_record_words(Neqn, net->nconst);
+        rc = reserve(sl->records, (size_t)Nsystems * pk.R * sizeof(float));
+        if (rc) return rc;
+        pk.rec = (float *)sl->records.ptr;
+    }
     if (opts.cost_hint && queue_kernel(net, opts)) {
         const int32_t *d_hint = opts.cost_hint;
         if (!opts.device_pointers) {
@@ -866,7 +897,7 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
             WIND_HIP(hipMemcpyAsync(sl->hint.ptr, opts.cost_hint, (size_t)Nsystems * 4, hipMemcpyHostToDevice, st));
             d_hint = (const int32_t *)sl->hint.ptr;
         }
-        rc = order_by_hint(sl->order, d_hint, Nsystems, st, &a.index);
+        rc = order_by_hint(sl->order, d_hint, Nsystems, st, &a.index, &pk);
         if (rc) return rc;
     } else if (auto_order(net, opts, Nsystems)) {
         rc = reserve(sl->hint, (size_t)Nsystems * 4);
@@ -874,9 +905,10 @@ int wind_integrate(const wind_network *net, float tnow, float tend, int n_integr
         if (rc) return rc;
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)sl->activity.ptr, 32);
         rc = activity_hints(c.num_cu, net, a, opts.method, (int32_t *)sl->hint.ptr, (unsigned *)sl->activity.ptr, st);
-        if (!rc) rc = order_by_hint(sl->order, (const int32_t *)sl->hint.ptr, Nsystems, st, &a.index);
+        if (!rc) rc = order_by_hint(sl->order, (const int32_t *)sl->hint.ptr, Nsystems, st, &a.index, &pk);
         if (rc) return rc;
     }
+    a.records = pk.rec;
     std::vector<TraceCopy> trace_back;
     if (opts.trace) {
         rc = setup_trace(sl->trace, opts, Nsystems, Neqn, &a.trace, &trace_back, st);
@@ -1378,13 +1410,14 @@ extern "C" int wind_integrate_batch(const wind_bucket *buckets, int nb, float tn
     return WIND_OK;
 }
 
-void wind_order_by_hint(const int32_t *hint, int n, int32_t *index, unsigned *count, hipStream_t st) {
+void wind_order_by_hint(const int32_t *hint, int n, int32_t *index, unsigned *count, hipStream_t st,
+                        const PackArgs *pk) {
     static_assert(kHintCountWords == kHintClasses, "one count per class");
     const int blocks = std::min(kHintBlocks, (n + 255) / 256);
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long *)count, kHintClasses / 2);
     hipLaunchKernelGGL(hint_count, dim3(blocks), dim3(256), 0, st, hint, n, count);
     hipLaunchKernelGGL(hint_starts, dim3(1), dim3(64), 0, st, count);
-    hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, hint, n, count, index);
+    hipLaunchKernelGGL(hint_place, dim3(blocks), dim3(256), 0, st, hint, n, count, index, pk ? *pk : PackArgs{});
 }
 
 // The lane kernels of compiled-in and generated networks and the dense

@@ -133,7 +133,14 @@ void wind_zero_counters(unsigned long long *p, int n, hipStream_t st);
 // Longest-first start order of n systems from device cost hints, written to
 // index[n] on `st`; count is device scratch of kHintCountWords 32-bit words.
 constexpr int kHintCountWords = 128;
-void wind_order_by_hint(const int32_t *hint, int n, int32_t *index, unsigned *count, hipStream_t st);
+// Optional packing of the systems into records in start order (hint_place).
+struct PackArgs {
+    const float *eq = nullptr, *cst = nullptr;
+    int N = 0, NC = 0, stride = 0, R = 0;
+    float *rec = nullptr;  // null: no packing
+};
+void wind_order_by_hint(const int32_t *hint, int n, int32_t *index, unsigned *count, hipStream_t st,
+                        const PackArgs *pk = nullptr);
 // Whether the kernels launched for (net, mode, kernel) take systems from a
 // work queue, so that a start order applies.
 bool wind_queue_kernel(const wind_network *net, int mode, int kernel);

@@ -42,6 +42,15 @@ struct IntegrateArgs {
     unsigned long long *counters;  // [0] work-queue head, [1] total sub-steps
     TraceArgs trace;               // per-attempt trace / replay (kernels that support it)
     int32_t pivot = 0;             // fast mode: partial pivoting in the LU solves (launchers pick the kernel)
+    // optional, with index (lane kernels): record q (lane_record_words(N,
+    // NCONST) floats) holds the state, the constants and the id (as int
+    // bits) of system index[q], packed in queue order, so that lanes read the
+    // systems they take contiguously instead of gathering them
+    const float *records = nullptr;
 };
+
+// Floats per packed record: state, constants and the system id, rounded up
+// to whole 32-byte sectors.
+constexpr int lane_record_words(int N, int NC) { return (N + NC + 1 + 7) / 8 * 8; }
 
 }  // namespace wind
