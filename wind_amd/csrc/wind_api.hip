@@ -1140,9 +1140,7 @@ std::vector<SolverChoice> candidates(const wind_network *net, const wind_options
     // the tile products of the dense solver: FP32 MFMA unless the caller names one
     const int sv = o.solver == WIND_SOLVER_AUTO ? WIND_SOLVER_MFMA_F32 : o.solver;
     const int blocks[4] = {64, 128, 256, 512};
-    // the register-resident solver first where it applies (SIE; the LDS
-    // solver's shapes are tried after it)
-    // Where the register-resident solver applies it is the only candidate:
+    // Where the register-resident solver applies (SIE) it is the only candidate:
     // the LDS solver rounds differently, and a bucket's results must not
     // depend on the tuning (every shape of one solver gives the same bits).
     if (o.method == WIND_SIE) {
