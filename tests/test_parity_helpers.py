@@ -1,4 +1,4 @@
-"""The float64 parity helpers (SURVEY 8(f)): an independent numpy float64
+"""The float64 parity helpers: an independent numpy float64
 integrator against the oracle's double build, and the Katz96 network's
 long-time attractor (the ionisation equilibrium of katz96.py:29-100)
 reached by the oracle and, on the GPU, by the library."""
