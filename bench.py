@@ -510,7 +510,7 @@ def dense_roofline(torch, api, W, nsys):
                           max_attempts=100000, flags=api.FLAG_DENSE_LDS)
     return {"workload": f"N = 157 mass-action, first {nsys} of {W.DENSE_NSYS} systems, t in [0, 1e9]",
             "sample": f"first {nsys} of the {W.DENSE_NSYS} systems of the N = 157 workload",
-            "kernel": "ma_dense_reg_integrate<10> (register-resident matrix, three systems per CU)",
+            "kernel": "ma_dense_reg_integrate<10> (register-resident matrix, four systems per CU)",
             "systems_per_s": nsys / (kms / 1e3),
             "kernel_ms": kms, "substeps": int(tot), "flops_per_substep": F,
             "us_per_substep_per_cu": kms * 1e3 / (float(tot) / 256.0),

@@ -221,7 +221,7 @@ enum wind_flags {
                                   the reference's Gauss-Jordan is unpivoted. */
     WIND_FLAG_DENSE_LDS = 8    /* fast mode, mass-action SIE with 49 <= N <= 160: keep the matrix in
                                   LDS (one system per CU at N = 157) instead of the default
-                                  register-resident solver (three systems per CU); results agree
+                                  register-resident solver (three or four systems per CU); results agree
                                   to rounding */
 };
 

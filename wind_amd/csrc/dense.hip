@@ -52,8 +52,8 @@ bool dense_reg_eligible(const MassActionDevice &d, const SolverChoice &sc) {
     if (hipGetDevice(&dev) != hipSuccess) return false;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
         return false;
-    // three workgroups per CU, or the LDS-resident solver is the better shape
-    return 3 * (dense_reg_lds_bytes(NT, d.R) + 256) <= (size_t)max_lds;
+    // the workgroups per CU it is built for, or the LDS-resident solver is the better shape
+    return (size_t)dense_reg_wgs(NT) * (dense_reg_lds_bytes(NT, d.R) + 256) <= (size_t)max_lds;
 }
 
 template <int NT>

@@ -4,11 +4,10 @@
 // the matrix A = I - hJ lives in the waves' registers instead of LDS: wave
 // (wi, wj) holds the 16x16 tiles (i, j) with i = wi, j = wj (mod 2), each in
 // the MFMA accumulator layout (lane l holds rows 4(l/16) + q, q = 0..3, of
-// column l % 16).  Without the matrix in LDS a workgroup needs under a
-// third of a CU's LDS and (at three waves per SIMD) a third of its
-// registers, so three systems share a CU and each one's serial parts (the
-// pivot chain of a diagonal block, the barriers) overlap with the others'
-// work.
+// column l % 16).  Without the matrix in LDS a workgroup needs a third
+// (N > 144: a quarter) of a CU's LDS and registers, so three (four) systems
+// share a CU and each one's serial parts (the pivot chain of a diagonal
+// block, the barriers) overlap with the others' work.
 //
 // Per pass (step-doubling sub-step) of an attempt:
 // * assembly: the per-reaction rate terms and f as in the LDS solver
@@ -39,10 +38,20 @@
 
 namespace wind {
 
+// Workgroups per CU the solver is built for: four at ten tiles per
+// dimension (N > 144: the compiler then keeps the kernel within 128 VGPRs,
+// spilling some, and the LDS holds a smaller assembly stage and no rate
+// constants, which are read from global memory), three below.  Measured at
+// N = 157: four per CU 3% faster than three; at N = 64 the four-per-CU
+// build was 8% slower.
+__host__ __device__ constexpr int dense_reg_wgs(int NT) { return NT >= 10 ? 4 : 3; }
+
 // Column blocks assembled per round: as many as keep the round's tiles
-// (NT x CB of 1 KB) within 30 KB of LDS, so that three workgroups fit in a
-// CU's LDS.
-__host__ __device__ constexpr int dense_reg_cb(int NT) { return NT * NT <= 30 ? NT : 30 / NT; }
+// (NT x CB of 1 KB) within 30 KB of LDS (20 KB at four workgroups per CU),
+// so that the workgroups fit in a CU's LDS.
+__host__ __device__ constexpr int dense_reg_cb(int NT) {
+    return NT * NT <= (dense_reg_wgs(NT) == 4 ? 20 : 30) ? NT : (dense_reg_wgs(NT) == 4 ? 20 : 30) / NT;
+}
 
 // Assembly stream words a thread loads at a time (streams are padded to a
 // multiple of it).
@@ -78,7 +87,7 @@ __host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R) {
     o.hdb = p;
     p = up(p + R);
     o.k = p;
-    p = up(p + R);
+    if (dense_reg_wgs(NT) == 3) p = up(p + R);  // rate constants in LDS
     o.ab = p;
     p = up(p + 2 * R);
     o.fptr = p;
@@ -155,7 +164,7 @@ __device__ inline void tile_matvec_store(const f32x4 &m, float v, float *dst, in
 // TRACE: per-attempt trace and replay through a.trace, as in the other
 // kernels.
 template <int NT, bool TRACE = false>
-__global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a, MassActionDevice net) {
+__global__ void __launch_bounds__(256, dense_reg_wgs(NT)) ma_dense_reg_integrate(IntegrateArgs a, MassActionDevice net) {
     constexpr int NP = 16 * NT, NL = (NT + 1) / 2, CB = dense_reg_cb(NT);
     constexpr int NR = (NT + CB - 1) / CB;
     extern __shared__ float lds[];
@@ -182,7 +191,7 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
     float *const rI = rv + 16 * wi;                        // rows of tile row i: + 32 * ta
     for (int r = t; r < R; r += 256) {
         ab[r] = make_int2(net.rx[4 * r], net.rx[4 * r + 1]);
-        kk[r] = net.k[r];
+        if (dense_reg_wgs(NT) == 3) kk[r] = net.k[r];
     }
     for (int i = t; i <= N; i += 256) fptr[i] = net.f_ptr[i];
     __shared__ int s_flag, s_sid;
@@ -212,7 +221,7 @@ __global__ void __launch_bounds__(256, 3) ma_dense_reg_integrate(IntegrateArgs a
         }
         for (int r = t; r < R; r += 256) {
             const int2 x = ab[r];
-            const float ks = kk[r] * scale;
+            const float ks = (dense_reg_wgs(NT) == 3 ? kk[r] : net.k[r]) * scale;
             const float ya = yv[x.x], yb = yv[x.y];
             w[r] = (ks * ya) * yb;
             hda[r] = hs * (ks * yb);

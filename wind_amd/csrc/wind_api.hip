@@ -1180,7 +1180,7 @@ int tune_bucket(Context &c, const wind_bucket &b, float tnow, float tend, int nf
     }
     const int N = b.net->N;
     // samples that fill the device several times over (the register-resident
-    // dense solver runs three systems per CU)
+    // dense solver runs three or four systems per CU)
     const int ns = std::min(b.nsys, N <= 16 ? 16384 : (N <= 64 ? 4096 : 12 * c.num_cu));
     const size_t bytes = (size_t)ns * N * sizeof(float);
     int rc = reserve(c.tune_state, bytes);
