@@ -231,12 +231,19 @@ def test_fast_lane_attempts_hhe6(method, tol, cap):
 # counts leaving some of a wave's tile slots unused) and on the LDS-resident
 # one with FLAG_DENSE_LDS; N = 32 on the LDS-resident one.
 DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (64, 32, 240, 0), (80, 16, 200, 0), (100, 12, 160, 0), (129, 8, 150, 0),
-                       (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
+                       (150, 8, 150, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
+
+# Networks with more reactions than the default 4 N: at N = 150, R = 1000
+# the register solver's LDS footprint leaves room for three workgroups per
+# CU, not four, so it runs its three-per-CU build at ten tiles.
+WIDE_R = {150: 1000}
 
 
 def dense_case(N, ns):
     if N == 157:
         rx, k = W.mass_action_network(157, seed=W.DENSE_NET_SEED)
+    elif N in WIDE_R:
+        rx, k = W.mass_action_network(N, seed=1000 + N, R=WIDE_R[N])
     else:
         rx, k = W.mixed_network(N)
     y0, sc = W.mass_action_systems(ns, N, seed=3000 + N, scale_range=W.MIXED_SCALE_RANGE)

@@ -42,25 +42,31 @@ WIND_DENSE(WIND_RK2, PREC_MFMA_BF16X3)
 // ---------------------------------------------------------------------------
 // Register-resident solver (kernels_dense_reg.h).
 
+// Workgroups per CU the register solver runs with on the current device for
+// network d (dense_reg_wgs_for), or 0 where it does not fit.
+static int dense_reg_wgs_dev(const MassActionDevice &d) {
+    int dev = 0, max_lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+        return 0;
+    return dense_reg_wgs_for((d.N + 15) / 16, d.R, (size_t)max_lds);
+}
+
 bool dense_reg_eligible(const MassActionDevice &d, const SolverChoice &sc) {
     if (sc.dense_kernel == 1 || !d.sc_ent) return false;
     if (sc.pivot || sc.ir_sweeps != 0) return false;
     if (sc.solver != WIND_SOLVER_AUTO && sc.solver != WIND_SOLVER_MFMA_F32) return false;
     const int NT = (d.N + 15) / 16;
     if (NT < kDenseRegMinNT || NT > kDenseRegMaxNT) return false;
-    int dev = 0, max_lds = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return false;
-    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
-        return false;
-    // the workgroups per CU it is built for, or the LDS-resident solver is the better shape
-    return (size_t)dense_reg_wgs(NT) * (dense_reg_lds_bytes(NT, d.R) + 256) <= (size_t)max_lds;
+    // three (or four) workgroups per CU must fit, else the LDS-resident solver is the better shape
+    return dense_reg_wgs_dev(d) != 0;
 }
 
-template <int NT>
+template <int NT, int WGS>
 int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
                         hipStream_t st) {
-    auto kfn = a.trace.capacity > 0 ? ma_dense_reg_integrate<NT, true> : ma_dense_reg_integrate<NT>;
-    const size_t lds = dense_reg_lds_bytes(NT, d.R);
+    auto kfn = a.trace.capacity > 0 ? ma_dense_reg_integrate<NT, WGS, true> : ma_dense_reg_integrate<NT, WGS>;
+    const size_t lds = dense_reg_lds_bytes(NT, d.R, WGS);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
@@ -78,13 +84,15 @@ int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateAr
 int launch_dense_reg(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
     switch ((d.N + 15) / 16) {
-    case 4: return launch_dense_reg_nt<4>(num_cu, d, a, sc, st);
-    case 5: return launch_dense_reg_nt<5>(num_cu, d, a, sc, st);
-    case 6: return launch_dense_reg_nt<6>(num_cu, d, a, sc, st);
-    case 7: return launch_dense_reg_nt<7>(num_cu, d, a, sc, st);
-    case 8: return launch_dense_reg_nt<8>(num_cu, d, a, sc, st);
-    case 9: return launch_dense_reg_nt<9>(num_cu, d, a, sc, st);
-    case 10: return launch_dense_reg_nt<10>(num_cu, d, a, sc, st);
+    case 4: return launch_dense_reg_nt<4, 3>(num_cu, d, a, sc, st);
+    case 5: return launch_dense_reg_nt<5, 3>(num_cu, d, a, sc, st);
+    case 6: return launch_dense_reg_nt<6, 3>(num_cu, d, a, sc, st);
+    case 7: return launch_dense_reg_nt<7, 3>(num_cu, d, a, sc, st);
+    case 8: return launch_dense_reg_nt<8, 3>(num_cu, d, a, sc, st);
+    case 9: return launch_dense_reg_nt<9, 3>(num_cu, d, a, sc, st);
+    case 10:
+        return dense_reg_wgs_dev(d) == 4 ? launch_dense_reg_nt<10, 4>(num_cu, d, a, sc, st)
+                                         : launch_dense_reg_nt<10, 3>(num_cu, d, a, sc, st);
     default: return wind_fail(WIND_ERR_UNSUPPORTED, "register-resident dense solver: N out of range");
     }
 }

@@ -5,8 +5,8 @@
 // (wi, wj) holds the 16x16 tiles (i, j) with i = wi, j = wj (mod 2), each in
 // the MFMA accumulator layout (lane l holds rows 4(l/16) + q, q = 0..3, of
 // column l % 16).  Without the matrix in LDS a workgroup needs a third
-// (N > 144: a quarter) of a CU's LDS and registers, so three (four) systems
-// share a CU and each one's serial parts (the pivot chain of a diagonal
+// (N > 144 and few enough reactions: a quarter) of a CU's LDS and
+// registers, so three (four) systems share a CU and each one's serial parts (the pivot chain of a diagonal
 // block, the barriers) overlap with the others' work.
 //
 // Per pass (step-doubling sub-step) of an attempt:
@@ -38,20 +38,17 @@
 
 namespace wind {
 
-// Workgroups per CU the solver is built for: four at ten tiles per
-// dimension (N > 144: the compiler then keeps the kernel within 128 VGPRs,
-// spilling some, and the LDS holds a smaller assembly stage and no rate
-// constants, which are read from global memory), three below.  Measured at
-// N = 157: four per CU 3% faster than three; at N = 64 the four-per-CU
-// build was 8% slower.
-__host__ __device__ constexpr int dense_reg_wgs(int NT) { return NT >= 10 ? 4 : 3; }
+// Workgroups per CU (WGS) the solver is built for: three, or four at ten
+// tiles per dimension (N > 144) when the network's LDS footprint allows it
+// (dense_reg_wgs_for): the compiler then keeps the kernel within 128 VGPRs,
+// spilling some, and the rate constants are read from global memory instead
+// of LDS.  Measured at N = 157: four per CU 3% faster than three; at N = 64
+// a four-per-CU build was 8% slower.
 
 // Column blocks assembled per round: as many as keep the round's tiles
-// (NT x CB of 1 KB) within 30 KB of LDS (20 KB at four workgroups per CU),
-// so that the workgroups fit in a CU's LDS.
-__host__ __device__ constexpr int dense_reg_cb(int NT) {
-    return NT * NT <= (dense_reg_wgs(NT) == 4 ? 20 : 30) ? NT : (dense_reg_wgs(NT) == 4 ? 20 : 30) / NT;
-}
+// (NT x CB of 1 KB) within 30 KB of LDS (20 KB at ten tiles, so that four
+// workgroups can fit in a CU's LDS).
+__host__ __device__ constexpr int dense_reg_cb(int NT) { return NT * NT <= 30 ? NT : (NT >= 10 ? 20 : 30) / NT; }
 
 // Assembly stream words a thread loads at a time (streams are padded to a
 // multiple of it).
@@ -65,7 +62,7 @@ struct DenseRegOffsets {
     int y, yw, y1, f, r, w, hda, hdb, k, ab, fptr, work, total;
 };
 
-__host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R) {
+__host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R, int WGS = 3) {
     const int NP = 16 * NT;
     auto up = [](int x) { return (x + 3) & ~3; };
     DenseRegOffsets o;
@@ -87,7 +84,7 @@ __host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R) {
     o.hdb = p;
     p = up(p + R);
     o.k = p;
-    if (dense_reg_wgs(NT) == 3) p = up(p + R);  // rate constants in LDS
+    if (WGS == 3) p = up(p + R);  // rate constants in LDS
     o.ab = p;
     p = up(p + 2 * R);
     o.fptr = p;
@@ -163,8 +160,8 @@ __device__ inline void tile_matvec_store(const f32x4 &m, float v, float *dst, in
 
 // TRACE: per-attempt trace and replay through a.trace, as in the other
 // kernels.
-template <int NT, bool TRACE = false>
-__global__ void __launch_bounds__(256, dense_reg_wgs(NT)) ma_dense_reg_integrate(IntegrateArgs a, MassActionDevice net) {
+template <int NT, int WGS, bool TRACE = false>
+__global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs a, MassActionDevice net) {
     constexpr int NP = 16 * NT, NL = (NT + 1) / 2, CB = dense_reg_cb(NT);
     constexpr int NR = (NT + CB - 1) / CB;
     extern __shared__ float lds[];
@@ -173,7 +170,7 @@ __global__ void __launch_bounds__(256, dense_reg_wgs(NT)) ma_dense_reg_integrate
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wi = wave >> 1, wj = wave & 1;
     const int lr = lane & 15, g = lane >> 4;
-    const DenseRegOffsets o = dense_reg_offsets(NT, R);
+    const DenseRegOffsets o = dense_reg_offsets(NT, R, WGS);
     float *y = lds + o.y, *yw = lds + o.yw, *y1 = lds + o.y1, *f = lds + o.f, *rv = lds + o.r;
     float *w = lds + o.w, *hda = lds + o.hda, *hdb = lds + o.hdb, *kk = lds + o.k;
     int2 *ab = reinterpret_cast<int2 *>(lds + o.ab);
@@ -191,7 +188,7 @@ __global__ void __launch_bounds__(256, dense_reg_wgs(NT)) ma_dense_reg_integrate
     float *const rI = rv + 16 * wi;                        // rows of tile row i: + 32 * ta
     for (int r = t; r < R; r += 256) {
         ab[r] = make_int2(net.rx[4 * r], net.rx[4 * r + 1]);
-        if (dense_reg_wgs(NT) == 3) kk[r] = net.k[r];
+        if (WGS == 3) kk[r] = net.k[r];
     }
     for (int i = t; i <= N; i += 256) fptr[i] = net.f_ptr[i];
     __shared__ int s_flag, s_sid;
@@ -221,7 +218,7 @@ __global__ void __launch_bounds__(256, dense_reg_wgs(NT)) ma_dense_reg_integrate
         }
         for (int r = t; r < R; r += 256) {
             const int2 x = ab[r];
-            const float ks = (dense_reg_wgs(NT) == 3 ? kk[r] : net.k[r]) * scale;
+            const float ks = (WGS == 3 ? kk[r] : net.k[r]) * scale;
             const float ya = yv[x.x], yb = yv[x.y];
             w[r] = (ks * ya) * yb;
             hda[r] = hs * (ks * yb);
@@ -519,6 +516,16 @@ __global__ void __launch_bounds__(256, dense_reg_wgs(NT)) ma_dense_reg_integrate
 }
 
 // LDS bytes of ma_dense_reg_integrate (dynamic part) for a network.
-inline size_t dense_reg_lds_bytes(int NT, int R) { return sizeof(float) * (size_t)dense_reg_offsets(NT, R).total; }
+inline size_t dense_reg_lds_bytes(int NT, int R, int WGS = 3) {
+    return sizeof(float) * (size_t)dense_reg_offsets(NT, R, WGS).total;
+}
+
+// The workgroups per CU to build for: four at ten tiles when four fit the
+// CU's LDS (max_lds bytes), else three when three fit, else 0 (the solver
+// does not apply).
+inline int dense_reg_wgs_for(int NT, int R, size_t max_lds) {
+    if (NT >= 10 && 4 * (dense_reg_lds_bytes(NT, R, 4) + 256) <= max_lds) return 4;
+    return 3 * (dense_reg_lds_bytes(NT, R, 3) + 256) <= max_lds ? 3 : 0;
+}
 
 }  // namespace wind
