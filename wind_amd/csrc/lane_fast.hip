@@ -162,10 +162,14 @@ int launch_lane_probe(int num_cu, const typename Net::Params &p, const Integrate
     return e == hipSuccess ? WIND_OK : wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
 }
 
-// Persistent blocks per CU.  Fewer lanes than the registers allow run
-// faster: on the 2^20-system hhe6 workload (SIE, tol 2e-6) a launch took
-// 70.3 / 61.6 / 66.3 / 71.8 ms with 1 / 2 / 3 / 4 blocks (waves per SIMD).
-constexpr int kLaneFastBlocksPerCU = 2;
+// Persistent blocks per CU, by method.  Fewer lanes than the registers
+// allow run faster.  On the 2^20-system hhe6 workload at tol 2e-6 (start
+// order from the probe) a launch took, with 1 / 2 / 3 / 4 blocks (waves per
+// SIMD): SIE 55.0 / 48.9 ms / - / - (70.3 / 61.6 / 66.3 / 71.8 ms with an
+// earlier kernel and no start order); RK2 242 / 278 / 320 / 368 ms (RK2's
+// sub-step counts run to 22x their mean, and a lane alone on its SIMD
+// finishes the longest systems sooner).
+constexpr int lane_fast_blocks_per_cu(int method) { return method == WIND_RK2 ? 1 : 2; }
 
 template <class Net, int METHOD>
 int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateArgs &a, const SolverChoice &sc,
@@ -177,7 +181,7 @@ int launch_lane_fast(int num_cu, const typename Net::Params &p, const IntegrateA
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
     if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));
-    per_cu = per_cu < kLaneFastBlocksPerCU ? per_cu : kLaneFastBlocksPerCU;
+    per_cu = per_cu < lane_fast_blocks_per_cu(METHOD) ? per_cu : lane_fast_blocks_per_cu(METHOD);
     const int grid = scaled_grid(num_cu, per_cu, sc, a.nsys, 256);
     hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, a, p);
     e = hipGetLastError();
