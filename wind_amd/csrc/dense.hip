@@ -49,7 +49,7 @@ static int dense_reg_wgs_dev(const MassActionDevice &d) {
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
         return 0;
-    return dense_reg_wgs_for((d.N + 15) / 16, d.R, (size_t)max_lds);
+    return dense_reg_wgs_for((d.N + 15) / 16, d.R, d.sc_cap, (size_t)max_lds);
 }
 
 bool dense_reg_eligible(const MassActionDevice &d, const SolverChoice &sc) {
@@ -66,7 +66,7 @@ template <int NT, int WGS>
 int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
                         hipStream_t st) {
     auto kfn = a.trace.capacity > 0 ? ma_dense_reg_integrate<NT, WGS, true> : ma_dense_reg_integrate<NT, WGS>;
-    const size_t lds = dense_reg_lds_bytes(NT, d.R, WGS);
+    const size_t lds = dense_reg_lds_bytes(NT, d.R, WGS, d.sc_cap);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return wind_fail(WIND_ERR_DEVICE, hipGetErrorString(e));

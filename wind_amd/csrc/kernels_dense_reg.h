@@ -45,24 +45,19 @@ namespace wind {
 // of LDS.  Measured at N = 157: four per CU 3% faster than three; at N = 64
 // a four-per-CU build was 8% slower.
 
-// Column blocks assembled per round: as many as keep the round's tiles
-// (NT x CB of 1 KB) within 30 KB of LDS (20 KB at ten tiles, so that four
-// workgroups can fit in a CU's LDS).
-__host__ __device__ constexpr int dense_reg_cb(int NT) { return NT * NT <= 30 ? NT : (NT >= 10 ? 20 : 30) / NT; }
-
 // Assembly stream words a thread loads at a time (streams are padded to a
 // multiple of it).
 constexpr int kDenseRegBatch = 4;
 
 // LDS layout of the register-resident solver (offsets in floats, 16-byte
 // aligned): per-system vectors, per-reaction rate terms, reactant indices
-// and rate constants, and a work area shared by the assembly stage and the
-// factorisation's panel buffers.  The term lists stay in global memory.
+// and rate constants, and a work area shared by the compact element buffer
+// of the assembly (cap floats) and the factorisation's panel buffers.  The term lists stay in global memory.
 struct DenseRegOffsets {
     int y, yw, y1, f, r, w, hda, hdb, k, ab, fptr, work, total;
 };
 
-__host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R, int WGS = 3) {
+__host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R, int WGS, int cap) {
     const int NP = 16 * NT;
     auto up = [](int x) { return (x + 3) & ~3; };
     DenseRegOffsets o;
@@ -90,8 +85,8 @@ __host__ __device__ inline DenseRegOffsets dense_reg_offsets(int NT, int R, int 
     o.fptr = p;
     p = up(p + NP + 1);
     o.work = p;
-    const int stage = NT * dense_reg_cb(NT) * 256, lu = (2 * NT + 1) * 256;
-    p = up(p + (stage > lu ? stage : lu));
+    const int lu = (2 * NT + 1) * 256;
+    p = up(p + (cap > lu ? cap : lu));
     o.total = p;
     return o;
 }
@@ -162,22 +157,21 @@ __device__ inline void tile_matvec_store(const f32x4 &m, float v, float *dst, in
 // kernels.
 template <int NT, int WGS, bool TRACE = false>
 __global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs a, MassActionDevice net) {
-    constexpr int NP = 16 * NT, NL = (NT + 1) / 2, CB = dense_reg_cb(NT);
-    constexpr int NR = (NT + CB - 1) / CB;
+    constexpr int NP = 16 * NT, NL = (NT + 1) / 2;
     extern __shared__ float lds[];
     const int N = net.N, R = net.R, t = threadIdx.x;
     const int lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wi = wave >> 1, wj = wave & 1;
     const int lr = lane & 15, g = lane >> 4;
-    const DenseRegOffsets o = dense_reg_offsets(NT, R, WGS);
+    const DenseRegOffsets o = dense_reg_offsets(NT, R, WGS, net.sc_cap);
     float *y = lds + o.y, *yw = lds + o.yw, *y1 = lds + o.y1, *f = lds + o.f, *rv = lds + o.r;
     float *w = lds + o.w, *hda = lds + o.hda, *hdb = lds + o.hdb, *kk = lds + o.k;
     int2 *ab = reinterpret_cast<int2 *>(lds + o.ab);
     int *fptr = reinterpret_cast<int *>(lds + o.fptr);
     const int *__restrict__ fent = net.f_ent;
     const int *__restrict__ scent = net.sc_ent;
-    const int *__restrict__ scoff = net.sc_off;
+    const unsigned *__restrict__ sctile = net.sc_tile;
     float *work = lds + o.work;
     float *Gbuf = work, *Cbuf = work + 256, *Wbuf = work + 256 * (NT + 1);
     // Per-lane bases of this wave's tiles (tile row i = 2 ta + wi, column
@@ -208,14 +202,12 @@ __global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs
         for (int ta = 0; ta < NL; ta++)
 #pragma unroll
             for (int tb = 0; tb < NL; tb++) M[ta][tb] = __builtin_nondeterministic_value(M[ta][tb]);
-        // the first words of round 0's term stream, loaded while the rate
-        // terms and f are formed
+        // the first words of the term stream, loaded while the rate terms
+        // and f are formed
         int en[B];
-        {
-            const int *sp = scent + (size_t)scoff[0] * 256 + t;
+        const int *sp = scent + t;
 #pragma unroll
-            for (int u = 0; u < B; u++) en[u] = sp[u * 256];
-        }
+        for (int u = 0; u < B; u++) en[u] = sp[u * 256];
         for (int r = t; r < R; r += 256) {
             const int2 x = ab[r];
             const float ks = (WGS == 3 ? kk[r] : net.k[r]) * scale;
@@ -244,33 +236,16 @@ __global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs
             }
             f[t] = acc;
         }
-        // The matrix, CB column blocks per round, through a stage buffer in
-        // tile order (tile (i, cb) of the round at (i * ncb + cb - cb0) * 256,
-        // each tile in the accumulator layout), so that an owner loads a tile
-        // with one 16-byte read per lane.  The stage starts as the identity;
-        // each thread then walks its term stream (the round's elements dealt
-        // to the 256 threads on the host, interleaved so that a batch of
-        // words is one coalesced load per word): an element is its terms
-        // summed in order in a register (from 1 on the diagonal), stored
-        // once at its stage offset.
-#pragma unroll
-        for (int rd = 0; rd < NR; rd++) {
-            const int cb0 = rd * CB, cb1 = cb0 + CB < NT ? cb0 + CB : NT, ncb = cb1 - cb0;
-            const int s0 = scoff[rd], len = scoff[rd + 1] - s0;
-            const int *sp = scent + (size_t)s0 * 256 + t;
-            f32x4 *S4 = reinterpret_cast<f32x4 *>(work);
-            if (rd > 0) {
-                // this round's first words, loaded before the stage fill
-#pragma unroll
-                for (int u = 0; u < B; u++) en[u] = sp[u * 256];
-            }
-            for (int q = t; q < NT * ncb * 64; q += 256) {
-                // chunk q: tile (i, cb0 + cl), lane l: rows 16 i + 4 (l >> 4) + c, column 16 (cb0 + cl) + (l & 15)
-                const int tile = q >> 6, l = q & 63, i = tile / ncb, cl = tile - i * ncb;
-                const int dc = 16 * (cb0 + cl) + (l & 15) - 16 * i - 4 * (l >> 4);
-                S4[q] = f32x4{dc == 0 ? 1.0f : 0.0f, dc == 1 ? 1.0f : 0.0f, dc == 2 ? 1.0f : 0.0f, dc == 3 ? 1.0f : 0.0f};
-            }
-            __syncthreads();
+        // The matrix's non-zero elements through a compact buffer: each
+        // thread walks its term stream (the elements dealt to the 256
+        // threads on the host, interleaved so that a batch of words is one
+        // coalesced load per word); an element is its terms summed in order
+        // in a register (from 1 on the diagonal), stored once at its offset.
+        // The owner lane of each tile element then reads it from the buffer
+        // through the tile's lane masks (1 on the diagonal, 0 elsewhere where
+        // the element is absent).
+        {
+            const int len = net.sc_len;
             float acc = 0.0f;
             bool start = true;
             for (int j = 0;;) {
@@ -291,19 +266,42 @@ __global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs
 #pragma unroll
                 for (int u = 0; u < B; u++) en[u] = sp[(j + u) * 256];
             }
-            __syncthreads();
+        }
+        __syncthreads();
+        // (an opaque offset per pass, so that the compiler does not keep
+        // every tile's words live across passes)
+        int tsel = 0;
+        asm volatile("" : "+s"(tsel));
+        const unsigned *tpass = sctile + tsel;
 #pragma unroll
-            for (int cl = 0; cl < CB; cl++) {
-                const int cb = rd * CB + cl;
-                if (cb < NT && wj == (cb & 1)) {
+        for (int ta = 0; ta < NL; ta++) {
+            const int i = 2 * ta + wi;
+            // this lane's word of each tile in the row: offset << 4 | the
+            // mask of its q present (a row of tiles at a time, so that few
+            // loads are in flight beside the tiles already filled)
+            unsigned wrow[NL];
 #pragma unroll
-                    for (int ta = 0; ta < NL; ta++) {
-                        const int i = 2 * ta + wi;
-                        if (i < NT) M[ta][cb >> 1] = S4[wi * ncb * 64 + lane + (2 * ta * ncb + cl) * 64];
-                    }
-                }
+            for (int tb = 0; tb < NL; tb++) {
+                const int jb = 2 * tb + wj;
+                wrow[tb] = (i < NT && jb < NT) ? tpass[(i * NT + jb) * 64 + lane] : 0u;
             }
-            __syncthreads();
+#pragma unroll
+            for (int tb = 0; tb < NL; tb++) {
+                const int jb = 2 * tb + wj;
+                if (i >= NT || jb >= NT) continue;
+                const unsigned wd = wrow[tb];
+                unsigned p = wd >> 4;
+                f32x4 m;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const bool has = ((wd >> q) & 1u) != 0;
+                    const float x = work[p];
+                    m[q] = has ? x : (i == jb && 4 * g + q == lr ? 1.0f : 0.0f);
+                    p += has ? 1u : 0u;
+                }
+                M[ta][tb] = m;
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
     };
 
@@ -516,16 +514,16 @@ __global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs
 }
 
 // LDS bytes of ma_dense_reg_integrate (dynamic part) for a network.
-inline size_t dense_reg_lds_bytes(int NT, int R, int WGS = 3) {
-    return sizeof(float) * (size_t)dense_reg_offsets(NT, R, WGS).total;
+inline size_t dense_reg_lds_bytes(int NT, int R, int WGS, int cap) {
+    return sizeof(float) * (size_t)dense_reg_offsets(NT, R, WGS, cap).total;
 }
 
 // The workgroups per CU to build for: four at ten tiles when four fit the
 // CU's LDS (max_lds bytes), else three when three fit, else 0 (the solver
-// does not apply).
-inline int dense_reg_wgs_for(int NT, int R, size_t max_lds) {
-    if (NT >= 10 && 4 * (dense_reg_lds_bytes(NT, R, 4) + 256) <= max_lds) return 4;
-    return 3 * (dense_reg_lds_bytes(NT, R, 3) + 256) <= max_lds ? 3 : 0;
+// does not apply).  cap: the network's compact element buffer (floats).
+inline int dense_reg_wgs_for(int NT, int R, int cap, size_t max_lds) {
+    if (NT >= 10 && 4 * (dense_reg_lds_bytes(NT, R, 4, cap) + 256) <= max_lds) return 4;
+    return 3 * (dense_reg_lds_bytes(NT, R, 3, cap) + 256) <= max_lds ? 3 : 0;
 }
 
 }  // namespace wind

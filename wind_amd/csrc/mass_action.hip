@@ -68,24 +68,32 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
         r_z.insert(r_z.end(), rows[i].begin(), rows[i].end());
         r_ptr[i + 1] = (int32_t)r_z.size();
     }
-    // Assembly streams of the register-resident dense solver.  Each matrix
-    // element (row, col) with Jacobian terms is its terms in the order
-    // dense_eval applies them (the row's reactions in sweep order, the
-    // reaction's first reactant before its second).  Per assembly round
-    // (dense_reg_cb(NT) column blocks) the round's elements are dealt to the
-    // workgroup's 256 threads, most terms first to the least-loaded thread;
-    // a thread's stream is its elements' terms, one word each:
-    //   (LDS float index of the term's h dJ value) << 16 | (stage offset of
+    // Assembly of the register-resident dense solver.  The matrix elements
+    // (row, col) with Jacobian terms live, once summed, in a compact LDS
+    // buffer (the others are 1 on the diagonal, 0 elsewhere), ordered by
+    // 16x16 tile (row-major over tiles), then by lane
+    // l = 16 ((row % 16) / 4) + col % 16 of the MFMA accumulator layout, then
+    // by that lane's accumulator register q = row % 4.  Per tile and lane,
+    // sc_tile[tile * 64 + l] = (offset of the lane's first element) << 4 |
+    // (mask of the q present), so that the owner lane finds element q at
+    // that offset plus the present q below it.  An element is its terms in
+    // the order dense_eval applies them (the row's reactions in sweep order,
+    // the reaction's first reactant before its second), summed from 1 on the
+    // diagonal; the elements are dealt to the workgroup's 256
+    // threads, most terms first to the least-loaded thread, and a thread's
+    // stream is its elements' terms, one word each:
+    //   (LDS float index of the term's h dJ value) << 16 | (buffer offset of
     //   the element) << 3 | (element on the diagonal) << 2 | (element's last
     //   term) << 1 | (reactant row: the term adds)
     // Streams are padded with zero words (which store nothing) to the
-    // round's longest, rounded up to kDenseRegBatch words, and interleaved:
-    // word j of thread t of round rd at sc_ent[(sc_off[rd] + j) * 256 + t].
+    // longest, rounded up to kDenseRegBatch words, and interleaved: word j
+    // of thread t at sc_ent[j * 256 + t].
     const int NT = (N + 15) / 16;
-    std::vector<int32_t> sc_off, sc_ent;
+    std::vector<int32_t> sc_ent;
+    std::vector<uint32_t> sc_tile;
+    int sc_len = 0, sc_cap = 0;
     if (NT >= kDenseRegMinNT && NT <= kDenseRegMaxNT) {
-        const DenseRegOffsets lo = dense_reg_offsets(NT, R);
-        const int CB = dense_reg_cb(NT), NRD = (NT + CB - 1) / CB;
+        const DenseRegOffsets lo = dense_reg_offsets(NT, R, 3, 0);
         bool ok = lo.hdb + R < 65536;
         std::map<std::pair<int, int>, std::vector<uint32_t>> elem;  // (row, col) -> words without last
         for (int i = 0; i < N && ok; i++)
@@ -97,40 +105,55 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
                     elem[{i, x[which]}].push_back((hd << 16) | ((x[which] == i ? 1u : 0u) << 2) | (uint32_t)neg);
                 }
             }
-        sc_off.push_back(0);
-        for (int rd = 0; rd < NRD && ok; rd++) {
-            const int cb0 = rd * CB, ncb = std::min(NT, cb0 + CB) - cb0;
-            std::vector<std::pair<int, std::vector<uint32_t>>> items;  // (count, words)
-            for (auto &kv : elem) {
-                const int row = kv.first.first, col = kv.first.second;
-                if (col / 16 < cb0 || col / 16 >= cb0 + ncb) continue;
-                const uint32_t dest = (uint32_t)(((row >> 4) * ncb + (col >> 4) - cb0) * 256 + 64 * ((row & 15) >> 2) +
-                                                 (row & 3) + 4 * (col & 15));
-                std::vector<uint32_t> w = kv.second;
-                for (auto &x : w) x |= dest << 3;
-                w.back() |= 2u;
-                items.push_back({(int)w.size(), std::move(w)});
-            }
-            std::stable_sort(items.begin(), items.end(),
-                             [](const auto &p1, const auto &p2) { return p1.first > p2.first; });
-            std::vector<std::vector<uint32_t>> per(256);
-            for (auto &it : items) {
-                size_t th = 0;
-                for (size_t u = 1; u < per.size(); u++)
-                    if (per[u].size() < per[th].size()) th = u;
-                per[th].insert(per[th].end(), it.second.begin(), it.second.end());
-            }
-            size_t len = kDenseRegBatch;
-            for (auto &p : per) len = std::max(len, (p.size() + kDenseRegBatch - 1) / kDenseRegBatch * kDenseRegBatch);
-            const size_t base = sc_ent.size();
-            sc_ent.resize(base + len * 256, 0);
-            for (int th = 0; th < 256; th++)
-                for (size_t j = 0; j < per[th].size(); j++) sc_ent[base + j * 256 + th] = (int32_t)per[th][j];
-            sc_off.push_back(sc_off.back() + (int32_t)len);
+        // the q present per (tile, lane), then the offsets
+        std::vector<uint32_t> mask4((size_t)NT * NT * 64, 0);
+        auto slot = [&](int row, int col, int *tl, int *q) {
+            *tl = ((row >> 4) * NT + (col >> 4)) * 64 + 16 * ((row & 15) >> 2) + (col & 15);
+            *q = row & 3;
+        };
+        for (auto &kv : elem) {
+            int tl, q;
+            slot(kv.first.first, kv.first.second, &tl, &q);
+            mask4[tl] |= 1u << q;
         }
+        sc_tile.resize(mask4.size());
+        uint32_t nnz = 0;
+        for (size_t u = 0; u < mask4.size(); u++) {
+            sc_tile[u] = nnz << 4 | mask4[u];
+            nnz += (uint32_t)__builtin_popcount(mask4[u]);
+        }
+        // (+1: an owner lane reads one place past its elements, unused)
+        sc_cap = (int)((nnz + 1 + 3) & ~3u);
+        ok = ok && sc_cap < 8192;
+        std::vector<std::pair<int, std::vector<uint32_t>>> items;  // (count, words)
+        for (auto &kv : elem) {
+            int tl, q;
+            slot(kv.first.first, kv.first.second, &tl, &q);
+            const uint32_t dest = (sc_tile[tl] >> 4) + (uint32_t)__builtin_popcount(mask4[tl] & ((1u << q) - 1));
+            std::vector<uint32_t> w = kv.second;
+            for (auto &x : w) x |= dest << 3;
+            w.back() |= 2u;
+            items.push_back({(int)w.size(), std::move(w)});
+        }
+        std::stable_sort(items.begin(), items.end(),
+                         [](const auto &p1, const auto &p2) { return p1.first > p2.first; });
+        std::vector<std::vector<uint32_t>> per(256);
+        for (auto &it : items) {
+            size_t th = 0;
+            for (size_t u = 1; u < per.size(); u++)
+                if (per[u].size() < per[th].size()) th = u;
+            per[th].insert(per[th].end(), it.second.begin(), it.second.end());
+        }
+        size_t len = kDenseRegBatch;
+        for (auto &p : per) len = std::max(len, (p.size() + kDenseRegBatch - 1) / kDenseRegBatch * kDenseRegBatch);
+        sc_ent.assign(len * 256, 0);
+        for (int th = 0; th < 256; th++)
+            for (size_t j = 0; j < per[th].size(); j++) sc_ent[j * 256 + th] = (int32_t)per[th][j];
+        sc_len = (int)len;
         if (!ok) {
-            sc_off.clear();
             sc_ent.clear();
+            sc_tile.clear();
+            sc_len = sc_cap = 0;
         }
     }
     MassActionDevice &d = n->ma;
@@ -146,14 +169,16 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     if (!rc) rc = upload((void **)&d.j_ent, j_ent.data(), sizeof(int32_t) * j_ent.size());
     if (!rc) rc = upload((void **)&d.r_ptr, r_ptr.data(), sizeof(int32_t) * r_ptr.size());
     if (!rc) rc = upload((void **)&d.r_z, r_z.data(), sizeof(int32_t) * r_z.size());
-    if (!rc && !sc_off.empty()) rc = upload((void **)&d.sc_off, sc_off.data(), sizeof(int32_t) * sc_off.size());
-    if (!rc && !sc_off.empty()) rc = upload((void **)&d.sc_ent, sc_ent.data(), sizeof(int32_t) * sc_ent.size());
+    if (!rc && !sc_ent.empty()) rc = upload((void **)&d.sc_ent, sc_ent.data(), sizeof(int32_t) * sc_ent.size());
+    if (!rc && !sc_ent.empty()) rc = upload((void **)&d.sc_tile, sc_tile.data(), sizeof(uint32_t) * sc_tile.size());
+    d.sc_len = sc_len;
+    d.sc_cap = sc_cap;
     return rc;
 }
 
 void mass_action_release(wind_network *n) {
     MassActionDevice &d = n->ma;
-    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.sc_off, d.sc_ent};
+    void *ptrs[] = {d.rx, d.k, d.f_ptr, d.f_ent, d.j_pos, d.j_ptr, d.j_ent, d.r_ptr, d.r_z, d.sc_ent, d.sc_tile};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = MassActionDevice();

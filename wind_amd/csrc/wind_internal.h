@@ -23,11 +23,14 @@ struct MassActionDevice {
     int32_t *j_ent = nullptr;  //        ((r*2 + use_db)*2) + (sign < 0)
     int32_t *r_ptr = nullptr;  // [N+1]  non-zeros by row (for residuals)
     int32_t *r_z = nullptr;    //        indices into j_pos
-    // Register-resident dense solver: the assembly rounds' per-thread term
-    // streams (built in mass_action_build, which documents the word format),
-    // round rd's word j of thread t at sc_ent[(sc_off[rd] + j) * 256 + t].
-    int32_t *sc_off = nullptr;  // [rounds + 1]
+    // Register-resident dense solver (built in mass_action_build, which
+    // documents the formats): the per-thread term streams, word j of thread
+    // t at sc_ent[j * 256 + t], sc_len words per thread; the per-tile
+    // element masks and offsets into the compact element buffer (12 words
+    // per 16x16 tile); sc_cap floats of that buffer.
     int32_t *sc_ent = nullptr;
+    uint32_t *sc_tile = nullptr;
+    int sc_len = 0, sc_cap = 0;
 };
 
 }  // namespace wind
