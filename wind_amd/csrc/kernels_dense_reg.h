@@ -6,16 +6,18 @@
 // the MFMA accumulator layout (lane l holds rows 4(l/16) + q, q = 0..3, of
 // column l % 16).  Without the matrix in LDS a workgroup needs a third
 // (N > 144 and few enough reactions: a quarter) of a CU's LDS and
-// registers, so three (four) systems share a CU and each one's serial parts (the pivot chain of a diagonal
-// block, the barriers) overlap with the others' work.
+// registers, so three (four) systems share a CU and each one's serial
+// parts (the pivot chain of a diagonal block, the barriers) overlap with
+// the others' work.
 //
 // Per pass (step-doubling sub-step) of an attempt:
 // * assembly: the per-reaction rate terms and f as in the LDS solver
 //   (kernels_dense.h, dense_eval, same order, so f and every matrix element
-//   round identically); the matrix is formed in an LDS stage a few
-//   16-column blocks at a time (the round's elements dealt to all 256
-//   threads on the host, each element its terms in sweep order), and the
-//   owners of those blocks' tiles load them into registers;
+//   round identically); the matrix elements with Jacobian terms are summed
+//   into a compact LDS buffer (the elements dealt to all 256 threads on the
+//   host, each element its terms in sweep order), and the owner lanes of
+//   the tiles read them into registers through per-lane offset and mask
+//   words (1 on the diagonal, 0 elsewhere where absent);
 // * block LU without pivoting, A = [I 0; A21 G I][A11 A12; 0 A22 - A21 G A12],
 //   G = A11^-1, per 16-column panel k, with two barriers:
 //     B: W_kj = G_k A_kj for the row panel (G_k from LDS, A_kj in the
