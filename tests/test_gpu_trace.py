@@ -235,8 +235,10 @@ DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (64, 32, 240, 0), (80, 16, 200, 0), (10
 
 # Networks with more reactions than the default 4 N: at N = 150, R = 1000
 # the register solver's LDS footprint leaves room for three workgroups per
-# CU, not four, so it runs its three-per-CU build at ten tiles.
-WIDE_R = {150: 1000}
+# CU, not four, so it runs its three-per-CU build at ten tiles; at N = 96,
+# R = 3600 the Jacobian has 8,343 non-zero elements, more than the register
+# solver's compact element buffer holds (8,191), so another kernel runs.
+WIDE_R = {96: 3600, 150: 1000}
 
 
 def dense_case(N, ns):
@@ -248,6 +250,31 @@ def dense_case(N, ns):
         rx, k = W.mixed_network(N)
     y0, sc = W.mass_action_systems(ns, N, seed=3000 + N, scale_range=W.MIXED_SCALE_RANGE)
     return rx, k, y0, sc
+
+
+def test_fast_dense_beyond_compact_buffer():
+    """A network whose Jacobian has more non-zero elements than the register
+    solver's compact buffer holds (N = 96, R = 3600: 8,343 > 8,191) runs
+    another fast kernel: every system finishes, and its final state is
+    within 1e-3 of the float oracle's run under the same controller and
+    tolerances (relative to max(|y|, 1e-6 of the system's total abundance);
+    the runs differ by rounding only)."""
+    N, ns, tend = 96, 8, 1.0e5
+    rx, k, y0, sc = dense_case(N, ns)
+    net = api.Network.mass_action(N, rx, k)
+    y = y0.copy()
+    st = np.zeros(ns, np.int32)
+    api.integrate(net, 0.0, tend, 1, sc, y, ns, 1e-10, 8e-5, mode=api.MODE_FAST, controller=api.CTRL_ADAPTIVE,
+                  max_attempts=100000, status_out=st)
+    yo = y0.copy()
+    O.gold_integrate(api.SIE, O.mass_action(N, rx, k), 0.0, tend, 1, sc, yo, ns, 1e-10, 8e-5, const_stride=1,
+                     threads=ns, controller=api.CTRL_ADAPTIVE)
+    assert (st == api.STATUS_OK).all()
+    a, b = y.reshape(ns, N).astype(np.float64), yo.reshape(ns, N).astype(np.float64)
+    floor = 1e-6 * np.abs(b).sum(axis=1, keepdims=True)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), floor)
+    print(f"N={N} R=3600 fast: max rel diff vs float oracle {rel.max():.2e}", flush=True)
+    assert rel.max() <= 1e-3
 
 
 @pytest.mark.parametrize("N,ns,cap,flags", DENSE_ATTEMPT_CASES)
