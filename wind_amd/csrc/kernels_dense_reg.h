@@ -250,8 +250,8 @@ __global__ void __launch_bounds__(256, WGS) ma_dense_reg_integrate(IntegrateArgs
         // coalesced load per word); an element is its terms summed in order
         // in a register (from 1 on the diagonal), stored once at its offset.
         // The owner lane of each tile element then reads it from the buffer
-        // through the tile's lane masks (1 on the diagonal, 0 elsewhere where
-        // the element is absent).
+        // through its per-lane offset and mask word (1 on the diagonal, 0
+        // elsewhere where the element is absent).
         {
             const int len = net.sc_len;
             float acc = 0.0f;
