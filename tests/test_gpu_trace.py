@@ -230,15 +230,16 @@ def test_fast_lane_attempts_hhe6(method, tol, cap):
 # counts per dimension 4 .. 10; N = 80, 100, 129 cover 5, 7 and 9, odd
 # counts leaving some of a wave's tile slots unused) and on the LDS-resident
 # one with FLAG_DENSE_LDS; N = 32 on the LDS-resident one.
-DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (64, 32, 240, 0), (80, 16, 200, 0), (100, 12, 160, 0), (129, 8, 150, 0),
-                       (150, 8, 150, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
+DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (60, 16, 200, 0), (64, 32, 240, 0), (80, 16, 200, 0), (100, 12, 160, 0),
+                       (129, 8, 150, 0), (150, 8, 150, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 # Networks with more reactions than the default 4 N: at N = 150, R = 1000
 # the register solver's LDS footprint leaves room for three workgroups per
-# CU, not four, so it runs its three-per-CU build at ten tiles; at N = 96,
+# CU, not four, so it runs its three-per-CU build at ten tiles; at N = 60,
+# R = 1000 likewise three, not six, at four tiles; at N = 96,
 # R = 3600 the Jacobian has 8,343 non-zero elements, more than the register
 # solver's compact element buffer holds (8,191), so another kernel runs.
-WIDE_R = {96: 3600, 150: 1000}
+WIDE_R = {60: 1000, 96: 3600, 150: 1000}
 
 
 def dense_case(N, ns):

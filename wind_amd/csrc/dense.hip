@@ -84,7 +84,9 @@ int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateAr
 int launch_dense_reg(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
     switch ((d.N + 15) / 16) {
-    case 4: return launch_dense_reg_nt<4, 3>(num_cu, d, a, sc, st);
+    case 4:
+        return dense_reg_wgs_dev(d) == 6 ? launch_dense_reg_nt<4, 6>(num_cu, d, a, sc, st)
+                                         : launch_dense_reg_nt<4, 3>(num_cu, d, a, sc, st);
     case 5: return launch_dense_reg_nt<5, 3>(num_cu, d, a, sc, st);
     case 6: return launch_dense_reg_nt<6, 3>(num_cu, d, a, sc, st);
     case 7: return launch_dense_reg_nt<7, 3>(num_cu, d, a, sc, st);
