@@ -356,7 +356,8 @@ def mixed_batch(torch, api, W, nsys, steps, plain, use_generated=True):
         n = len(b["scale"])
         y0 = torch.from_numpy(b["y0"]).to(dev)
         table = api.Network.mass_action(b["N"], b["rx"], b["k"])
-        net, kind = table, "table (dense LDS solver)"
+        net, kind = table, ("table (register-resident dense solver)" if b["N"] in (32, 64, 157)
+                            else "table (dense LDS solver)")
         plugin = os.path.join(ROOT, "wind_amd", "lib", "gen", f"mixed{b['N']}", "libwind_net.so")
         if use_generated and b["N"] in W.GENERATED_MIXED and os.path.exists(plugin):
             net, kind = api.Network.plugin(plugin), "generated (lane per system)"

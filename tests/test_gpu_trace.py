@@ -230,7 +230,7 @@ def test_fast_lane_attempts_hhe6(method, tol, cap):
 # counts per dimension 4 .. 10; N = 80, 100, 129 cover 5, 7 and 9, odd
 # counts leaving some of a wave's tile slots unused) and on the LDS-resident
 # one with FLAG_DENSE_LDS; N = 32 on the LDS-resident one.
-DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (60, 16, 200, 0), (64, 32, 240, 0), (80, 16, 200, 0), (100, 12, 160, 0),
+DENSE_ATTEMPT_CASES = [(32, 48, 400, 0), (32, 48, 400, api.FLAG_DENSE_LDS), (60, 16, 200, 0), (64, 32, 240, 0), (80, 16, 200, 0), (100, 12, 160, 0),
                        (129, 8, 150, 0), (150, 8, 150, 0), (157, 8, 150, 0), (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 # Networks with more reactions than the default 4 N: at N = 150, R = 1000
@@ -333,7 +333,7 @@ def test_fast_lane_replay_hhe6_sie():
     whole_run_parity(api.SIE, O.hhe6(p), c, y0, n, 6, tr, y, tol, "hhe6 fast replay SIE", t)
 
 
-DENSE_CASES = [(32, 96, 0, 0), (64, 32, 240, 0), (100, 12, 240, 0), (157, 8, 150, 0),
+DENSE_CASES = [(32, 96, 0, 0), (32, 96, 0, api.FLAG_DENSE_LDS), (64, 32, 240, 0), (100, 12, 240, 0), (157, 8, 150, 0),
                (157, 8, 150, api.FLAG_DENSE_LDS)]
 
 

@@ -57,7 +57,7 @@ bool dense_reg_eligible(const MassActionDevice &d, const SolverChoice &sc) {
     if (sc.pivot || sc.ir_sweeps != 0) return false;
     if (sc.solver != WIND_SOLVER_AUTO && sc.solver != WIND_SOLVER_MFMA_F32) return false;
     const int NT = (d.N + 15) / 16;
-    if (NT < kDenseRegMinNT || NT > kDenseRegMaxNT) return false;
+    if (!dense_reg_tiles(NT)) return false;
     // three (or four) workgroups per CU must fit, else the LDS-resident solver is the better shape
     return dense_reg_wgs_dev(d) != 0;
 }
@@ -84,6 +84,9 @@ int launch_dense_reg_nt(int num_cu, const MassActionDevice &d, const IntegrateAr
 int launch_dense_reg(int num_cu, const MassActionDevice &d, const IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st) {
     switch ((d.N + 15) / 16) {
+    case 2:
+        return dense_reg_wgs_dev(d) == 8 ? launch_dense_reg_nt<2, 8>(num_cu, d, a, sc, st)
+                                         : launch_dense_reg_nt<2, 3>(num_cu, d, a, sc, st);
     case 4:
         return dense_reg_wgs_dev(d) == 6 ? launch_dense_reg_nt<4, 6>(num_cu, d, a, sc, st)
                                          : launch_dense_reg_nt<4, 3>(num_cu, d, a, sc, st);

@@ -41,13 +41,14 @@
 namespace wind {
 
 // Workgroups per CU (WGS) the solver is built for: three; four at ten tiles
-// per dimension (N > 144) and six at four tiles (N <= 64) when the
-// network's LDS footprint allows it (dense_reg_wgs_for).  Built for more
-// than three, the kernel reads the rate constants from global memory
-// instead of LDS and the compiler keeps it within 512 / WGS VGPRs (at ten
-// tiles spilling some).  Measured: N = 157 four per CU 3% faster than
-// three; N = 64 six per CU 6% faster than three (five, the register-
-// limited occupancy of the three-per-CU build).
+// per dimension (N > 144), six at four tiles (N <= 64) and eight at two
+// (N <= 32) when the network's LDS footprint allows it (dense_reg_wgs_for).
+// Built for more than three, the kernel reads the rate constants from
+// global memory instead of LDS and the compiler keeps it within 512 / WGS
+// VGPRs (at ten tiles spilling some).  Measured: N = 157 four per CU 3%
+// faster than three; N = 64 six per CU 6% faster than three (five, the
+// register-limited occupancy of the three-per-CU build); N = 32 at eight
+// per CU 8% faster than the LDS solver.
 
 // Assembly stream words a thread loads at a time (streams are padded to a
 // multiple of it).
@@ -544,12 +545,12 @@ inline size_t dense_reg_lds_bytes(int NT, int R, int WGS, int cap) {
     return sizeof(float) * (size_t)dense_reg_offsets(NT, R, WGS, cap).total;
 }
 
-// The workgroups per CU to build for: four at ten tiles and six at four
-// tiles when they fit the CU's LDS (max_lds bytes), else three when three
-// fit, else 0 (the solver does not apply).  cap: the network's compact
-// element buffer (floats).
+// The workgroups per CU to build for: four at ten tiles, six at four and
+// eight at two when they fit the CU's LDS (max_lds bytes), else three when
+// three fit, else 0 (the solver does not apply).  cap: the network's
+// compact element buffer (floats).
 inline int dense_reg_wgs_for(int NT, int R, int cap, size_t max_lds) {
-    const int want = NT >= 10 ? 4 : NT == 4 ? 6 : 3;
+    const int want = NT >= 10 ? 4 : NT == 4 ? 6 : NT == 2 ? 8 : 3;
     if (want != 3 && want * (dense_reg_lds_bytes(NT, R, want, cap) + 256) <= max_lds) return want;
     return 3 * (dense_reg_lds_bytes(NT, R, 3, cap) + 256) <= max_lds ? 3 : 0;
 }

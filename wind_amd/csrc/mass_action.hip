@@ -92,7 +92,7 @@ int mass_action_build(wind_network *n, int N, int R, const int32_t *rx, const fl
     std::vector<int32_t> sc_ent;
     std::vector<uint32_t> sc_tile;
     int sc_len = 0, sc_cap = 0;
-    if (NT >= kDenseRegMinNT && NT <= kDenseRegMaxNT) {
+    if (dense_reg_tiles(NT)) {
         const DenseRegOffsets lo = dense_reg_offsets(NT, R, 3, 0);
         bool ok = lo.hdb + R < 65536;
         std::map<std::pair<int, int>, std::vector<uint32_t>> elem;  // (row, col) -> words without last

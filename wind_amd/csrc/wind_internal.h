@@ -87,7 +87,10 @@ int launch_dense(int num_cu, const wind::MassActionDevice &d, const wind::Integr
 // Register-resident dense SIE solver (dense.hip): tiles per dimension it is
 // built for, whether it takes this network and choice (fast mode, FP32
 // tiles, no refinement, two workgroups' LDS per CU), and its launch.
-constexpr int kDenseRegMinNT = 4, kDenseRegMaxNT = 10;
+constexpr int kDenseRegMinNT = 2, kDenseRegMaxNT = 10;
+// Tiles per dimension at which the register-resident solver is used: at
+// three (33 <= N <= 48) it measured 7-10% slower than the LDS solver.
+constexpr bool dense_reg_tiles(int NT) { return NT >= kDenseRegMinNT && NT <= kDenseRegMaxNT && NT != 3; }
 bool dense_reg_eligible(const wind::MassActionDevice &d, const SolverChoice &sc);
 int launch_dense_reg(int num_cu, const wind::MassActionDevice &d, const wind::IntegrateArgs &a, const SolverChoice &sc,
                      hipStream_t st);
